@@ -1,0 +1,177 @@
+/*
+ * uqk.h -- C ABI of the MI355X-native IPM moment-solver hot path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and returns
+ * 0 on success or a negative error code (uqk_last_error() holds the message).
+ * Device-pointer entry points expect buffers in HBM on the context's device;
+ * *_host entry points accept host buffers and copy them.
+ *
+ * Host array layouts are the reference's:
+ *   per-cell moments u / duals lambda : Matrix(S, M) column-major, state fastest,
+ *                                       element (s, i) at [i*S + s]; cells contiguous
+ *                                       (code/src/matrix.cpp:107, momentsolver.h:24).
+ *   per-cell quadrature values uq     : Matrix(S, Q) column-major, (s, k) at [k*S + s]
+ *                                       (momentsolver.cpp:50).
+ *   basis tables phi, phiw            : (k, i) at [k*M + i]; phi = Closure::_phiTilde,
+ *                                       phiw = Closure::_phiTildeWf (closure.cpp:45-66).
+ */
+#ifndef UQK_H
+#define UQK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct uqk_ctx uqk_ctx;
+
+/* Closure kinds: Closure::Create (closure.cpp:94-131). */
+enum {
+    UQK_BOUNDED_BARRIER = 0,           /* BoundedBarrier (boundedbarrier.cpp)            */
+    UQK_EULER1D = 1,                   /* EulerClosure (eulerclosure.cpp)                */
+    UQK_EULER2D = 2,                   /* EulerClosure2D (eulerclosure2d.cpp)            */
+    UQK_SHALLOW_WATER2D = 3,           /* ShallowWaterClosure2D (shallowwaterclosure2d)  */
+    UQK_SHALLOW_WATER2D_BARRIER = 4    /* SW2D with a log barrier on h (no reference)    */
+};
+
+/* Arithmetic of the Hessian assembly. */
+enum {
+    UQK_FP64 = 0,   /* everything in FP64 (the plain path)                        */
+    UQK_MIXED = 1   /* Hessian on MFMA in reduced precision, FP64 Cholesky,
+                       FP64 gradient/residual/accept test                         */
+};
+
+/* Per-cell closure status (replaces the reference's exit(EXIT_FAILURE)). */
+enum {
+    UQK_STATUS_CONVERGED = 0,
+    UQK_STATUS_MAX_ITERATIONS = 1,     /* closure.cpp:191-192                     */
+    UQK_STATUS_LINE_SEARCH = 2,        /* closure.cpp:180-183                     */
+    UQK_STATUS_NOT_SPD = 3,            /* vectorspace.cpp:166-170 (dposv info>0)  */
+    UQK_STATUS_NONFINITE = 4           /* accepted a non-finite trial residual    */
+};
+
+/* Problem kinds for the flux / time step (Problem::Create, problem.cpp:18-42). */
+enum { UQK_BURGERS = 0, UQK_EULER2D_PROBLEM = 1, UQK_SHALLOW_WATER2D_PROBLEM = 2 };
+/* Flux flavour: the reference's (Burgers upwind / Euler2D, SW2D HLL) or the
+ * Lax-Friedrichs (Burgers) / Rusanov (2D) kinetic flux. */
+enum { UQK_FLUX_REFERENCE = 0, UQK_FLUX_LAX_FRIEDRICHS = 1 };
+/* Boundary types (BoundaryType in the reference's mesh.h). */
+enum { UQK_BC_NONE = 0, UQK_BC_DIRICHLET = 1, UQK_BC_NOSLIP = 2, UQK_BC_SWWALL = 3 };
+
+typedef struct {
+    int32_t closure;            /* UQK_BOUNDED_BARRIER ...                        */
+    int32_t n_states;           /* S                                              */
+    int32_t n_moments;          /* M = nTotal of the finest level                 */
+    int32_t n_quad;             /* Q = nQTotal                                    */
+    int32_t dim_xi;             /* d, number of uncertain dimensions              */
+    double gamma;               /* Euler adiabatic exponent                       */
+    double u_minus, u_plus;     /* bounded-barrier bounds                         */
+    double gravity;             /* shallow water g                                */
+    double barrier;             /* SW2D barrier strength                          */
+    double alpha;               /* Newton damping (closure-specific default)      */
+    double epsilon;             /* convergence threshold on ||grad L||_2          */
+    int32_t max_iterations;     /* Newton iterations (1 = one-shot IPM)           */
+    int32_t max_halvings;       /* line-search halvings (reference: 1000)         */
+    int32_t du_exact;           /* Euler1D: 1 exact DU, 0 DU(2,2) as shipped      */
+    int32_t safe_line_search;   /* 1: SolveClosureSafe semantics                  */
+    int32_t precision;          /* UQK_FP64 or UQK_MIXED                          */
+    const double* phi;          /* host, Q x M                                    */
+    const double* phiw;         /* host, Q x M                                    */
+    const double* xi;           /* host, Q x d quadrature nodes (for sources)     */
+    int32_t device;             /* HIP device ordinal                             */
+} uqk_desc;
+
+typedef struct {
+    int32_t problem;            /* UQK_BURGERS ...                                */
+    int32_t flux;               /* UQK_FLUX_REFERENCE or UQK_FLUX_LAX_FRIEDRICHS  */
+    double cfl;
+    double gamma;
+    double gravity;
+    double friction;            /* SW2D bed friction base coefficient (0 = off)   */
+    int32_t friction_dim;       /* xi dimension scaling the friction, -1 = none   */
+} uqk_problem;
+
+/* Mesh arrays on the device (see uqk_mesh_upload for host arrays).
+ * neighbor id == n_cells marks a boundary edge (mesh2d.cpp:194). */
+typedef struct {
+    int64_t n_cells;
+    int32_t n_edges;            /* edges per cell: 2 (1D lines) or 4 (quads)      */
+    int32_t dim;                /* 1 or 2                                         */
+    const int32_t* neighbor;    /* n_cells * n_edges                              */
+    const double* unit_normal;  /* n_cells * n_edges * dim                        */
+    const double* normal;       /* n_cells * n_edges * dim, scaled by edge length */
+    const double* area;         /* n_cells                                        */
+    const double* dx_param;     /* n_cells: maxEdge / area (ComputeDt argument)   */
+    const int32_t* bc;          /* n_cells, UQK_BC_*                              */
+    const double* boundary_normal; /* n_cells * dim                               */
+} uqk_mesh;
+
+/* Context: Closure::Create + the Closure constructor tables
+ * (closure.cpp:16-84, :94-131).  Uploads phi/phiw, picks kernels. */
+int uqk_create(const uqk_desc* desc, uqk_ctx** out);
+int uqk_destroy(uqk_ctx* ctx);
+const char* uqk_last_error(const uqk_ctx* ctx);
+/* Launch on a caller-provided hipStream_t (NULL = the context's own stream). */
+int uqk_set_stream(uqk_ctx* ctx, void* hip_stream);
+int uqk_synchronize(uqk_ctx* ctx);
+
+/* Closure::SolveClosure / SolveClosureSafe (closure.cpp:133-193, :297-339),
+ * batched over cells: the per-cell loop of momentsolver.cpp:83-89.
+ * lambda: in = warm start, out = solution.  ntotal_cell: per-cell nTotal
+ * (adaptive levels, momentsolver.cpp:88) or NULL for M.  status/iters may be NULL.
+ * Device pointers.  Returns the number of cells with nonzero status (>= 0). */
+int64_t uqk_closure_solve(uqk_ctx* ctx, int64_t n_cells, const int32_t* ntotal_cell, const double* u,
+                          double* lambda, int32_t* status, int32_t* iters);
+/* Same with host pointers (copies in and out). */
+int64_t uqk_closure_solve_host(uqk_ctx* ctx, int64_t n_cells, const int32_t* ntotal_cell, const double* u,
+                               double* lambda, int32_t* status, int32_t* iters);
+
+/* uQ_j = U(EvaluateLambdaOnPE(lambda_j)) (momentsolver.cpp:99-101,
+ * closure.cpp:217-230 with the U(Matrix) form).  Device pointers. */
+int uqk_eval_uq(uqk_ctx* ctx, int64_t n_cells, const int32_t* ntotal_cell, const double* lambda, double* uq);
+/* u_j = uQ_j * phiw over the cell's nTotal moments, rest zero
+ * (momentsolver.cpp:120-124, vectorspace.cpp:174-185).  Device pointers. */
+int uqk_reconstruct(uqk_ctx* ctx, int64_t n_cells, const int32_t* ntotal_cell, const double* uq, double* u);
+/* dt = min_j Problem::ComputeDt(uQ_j, maxEdge_j/area_j) (momentsolver.cpp:127-133).
+ * Result written to *dt_host. */
+int uqk_compute_dt(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const double* uq, double* dt_host);
+/* ExplicitEuler::Advance (expliciteuler.cpp:8-75) with MomentSolver::numFlux
+ * (momentsolver.cpp:413-416): unew = u - dt/A * sum_edges G(uQ_j, uQ_nb) phiw. */
+int uqk_advance(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const int32_t* ntotal_cell,
+                double dt, const double* u, const double* uq, double* unew);
+/* One MomentSolver::Solve step body (momentsolver.cpp:83-152): closure, uQ,
+ * reconstruction, dt, advance, residual.  u, lambda updated in place (device).
+ * Writes dt and the residual of the state-0 mean.  Returns the number of
+ * cells whose closure failed (>= 0) or an error code. */
+int64_t uqk_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const int32_t* ntotal_cell,
+                 double* u, double* lambda, double* uq, double* u_scratch, int32_t* status, int32_t* iters,
+                 double* dt_out, double* residual_out);
+/* WriteMeanAndVar (momentsolver.cpp:302-328): mv is (2S) x n_cells,
+ * (f, j) at [j*2S + f].  Device pointers. */
+int uqk_mean_var(uqk_ctx* ctx, int64_t n_cells, const int32_t* ntotal_cell, const double* lambda, double* mv);
+
+/* ---- host utilities (no GPU work) ------------------------------------- */
+/* Gauss-Legendre rule, nodes ascending (legendre.cpp:9-28, polynomial.cpp:7-16). */
+int uqk_gauss_legendre(int32_t n, double* x, double* w);
+/* Multi-index set of the moment basis (settings.cpp:242-266): idx is
+ * (count x d), dimension 0 fastest; ntotal_per_level[l] = nTotal of level l.
+ * Returns the count, or -1 if it exceeds cap. */
+int32_t uqk_index_set(int32_t d, int32_t n_levels, const int32_t* degrees, int32_t max_degree, int32_t cap,
+                      int32_t* idx, int32_t* ntotal_per_level);
+/* Tensor Gauss-Legendre grid and the Closure tables phi / phiw
+ * (tensorizedquadrature.cpp:27-74, closure.cpp:45-66).  Returns Q. */
+int32_t uqk_basis_tables(int32_t d, int32_t q1, int32_t m, const int32_t* idx, double* xi, double* wq, double* phi,
+                         double* phiw);
+
+/* Milliseconds spent in the closure kernel by the last uqk_closure_solve /
+ * uqk_step call, timed with HIP events on the launch stream. */
+double uqk_last_closure_ms(const uqk_ctx* ctx);
+/* Totals of the last closure call: Newton steps (Hessian factorisations) and
+ * gradient evaluations (line-search trials included) over all cells. */
+int uqk_last_closure_stats(const uqk_ctx* ctx, int64_t* newton_steps, int64_t* gradient_evaluations);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

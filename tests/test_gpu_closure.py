@@ -1,0 +1,161 @@
+"""GPU Newton closure (through the C ABI) against the FP64 CPU oracle."""
+import numpy as np
+import pytest
+
+from uqcreator_amd import native, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    import torch
+
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+
+
+def _run_gpu(cl, u, lam0, nt=None):
+    import torch
+
+    n = u.shape[0]
+    du, dl = _dev(u), _dev(lam0)
+    st = torch.zeros(n, dtype=torch.int32, device="cuda")
+    it = torch.zeros(n, dtype=torch.int32, device="cuda")
+    dnt = None if nt is None else _dev(np.asarray(nt, dtype=np.int32))
+    nfail = cl.solve(du, dl, st, it, nt_cell=dnt)
+    return nfail, dl.cpu().numpy(), st.cpu().numpy(), it.cpu().numpy()
+
+
+def test_euler1d_closure_matches_oracle(oracle):
+    b = workloads.closure_basis()
+    u, lam_star, lam0 = (t.numpy() for t in workloads.closure_euler1d(4096, basis=b, seed=11))
+    print("case: Euler1D S=3 M=15 Q=49, 4096 cells", flush=True)
+    cl = native.GpuClosure(native.EULER1D, b, epsilon=1e-10, du_exact=1, safe_line_search=1)
+    nfail, lam, st, it = _run_gpu(cl, u, lam0)
+    c = oracle.closure(oracle.EULER1D, eps=1e-10, du_exact=1, safe=1)
+    lam_o = lam0.copy()
+    st_o, it_o = oracle.solve_closure(c, b.phi, b.phiw, u, lam_o)
+    assert nfail == 0 and (st == 0).all() and (st_o == 0).all()
+    # identical algorithm in FP64: iteration counts agree (up to one at the tolerance edge)
+    assert np.abs(it - it_o).max() <= 1 and (it == it_o).mean() > 0.99
+    scale = np.abs(lam_star).max(axis=1, keepdims=True)
+    assert (np.abs(lam - lam_o) / scale).max() < 1e-8
+    assert (np.abs(lam - lam_star) / scale).max() < 1e-8
+    for j in range(0, 4096, 257):
+        assert np.linalg.norm(oracle.gradient(c, b.phi, b.phiw, lam[j], u[j])) < 1e-10
+
+
+def test_euler1d_unsafe_and_one_shot(oracle):
+    b = workloads.closure_basis()
+    u, lam_star, lam0 = (t.numpy() for t in workloads.closure_euler1d(512, basis=b, seed=5))
+    for kw, okw in [(dict(safe_line_search=0), dict(safe=0)), (dict(max_iterations=1), dict(max_iter=1)),
+                    (dict(du_exact=0, safe_line_search=1), dict(du_exact=0, safe=1))]:
+        print("case:", kw, flush=True)
+        cl = native.GpuClosure(native.EULER1D, b, **{"du_exact": 1, **kw})
+        nfail, lam, st, it = _run_gpu(cl, u, lam0)
+        c = oracle.closure(oracle.EULER1D, **{"du_exact": 1, **okw})
+        lam_o = lam0.copy()
+        st_o, it_o = oracle.solve_closure(c, b.phi, b.phiw, u, lam_o)
+        np.testing.assert_array_equal(st, st_o)
+        ok = st == 0
+        assert nfail == (~ok).sum()
+        scale = np.abs(lam_star).max(axis=1, keepdims=True)
+        if kw.get("max_iterations") == 1:
+            assert (it == 1).all()
+            assert (np.abs(lam - lam_o) / scale).max() < 1e-10
+        else:
+            assert (np.abs(lam - lam_o)[ok] / scale[ok]).max() < 1e-8
+
+
+def _maxwellian_duals(rng, n, b, S, ds, cons_fn, amp):
+    lam = np.zeros((n, b.M, S))
+    for j in range(n):
+        lam[j, 0] = ds(cons_fn(rng))
+        lam[j, 1:] = rng.normal(0, amp, (b.M - 1, S)) * np.abs(lam[j, 0])[None, :] * 0.02
+    return lam
+
+
+def _moments(oracle, c, b, lam):
+    uq = oracle.eval_uq(c, b.phi, lam)
+    return oracle.reconstruct(c.S, b.phiw, np.ascontiguousarray(uq))
+
+
+@pytest.mark.parametrize("kind,d,deg,q,extra", [
+    (native.BOUNDED_BARRIER, 1, 5, 12, dict(u_minus=2.9, u_plus=12.1)),
+    (native.EULER2D, 2, 3, 5, {}),
+    (native.SHALLOW_WATER2D, 2, 3, 5, {}),
+    (native.SHALLOW_WATER2D_BARRIER, 3, 4, 5, dict(barrier=0.01)),
+])
+def test_other_closures_match_oracle(oracle, kind, d, deg, q, extra):
+    b = native.Basis(d, [deg], q)
+    S = native.STATES[kind]
+    print("case: kind", kind, "M", b.M, "Q", b.Q, flush=True)
+    rng = np.random.default_rng(kind)
+    okind = {0: oracle.BB, 2: oracle.EULER2D, 3: oracle.SW2D, 4: oracle.SW2D_BARRIER}[kind]
+    c = oracle.closure(okind, eps=1e-9, safe=1, **{k: v for k, v in extra.items()})
+    n = 256 if b.M * S < 60 else 64
+    if kind == native.BOUNDED_BARRIER:
+        lam_star = rng.normal(0, 0.3, (n, b.M, 1))
+        lam_star[:, 0] = rng.uniform(-2, 2, (n, 1))
+    elif kind == native.EULER2D:
+        def cons(r):
+            rho, vx, vy, p = r.uniform(0.5, 2), r.uniform(-1, 1), r.uniform(-1, 1), r.uniform(0.5, 2)
+            return np.array([rho, rho * vx, rho * vy, p / 0.4 + 0.5 * rho * (vx * vx + vy * vy)])
+        lam_star = _maxwellian_duals(rng, n, b, 4, lambda x: oracle.DS(c, x), cons, 1.0)
+    else:
+        def cons(r):
+            h, vx, vy = r.uniform(1, 3), r.uniform(-1, 1), r.uniform(-1, 1)
+            return np.array([h, h * vx, h * vy])
+        lam_star = _maxwellian_duals(rng, n, b, 3, lambda x: oracle.DS(c, x), cons, 1.0)
+    u = _moments(oracle, c, b, lam_star)
+    lam0 = np.zeros_like(lam_star)
+    for j in range(n):
+        lam0[j, 0] = oracle.DS(c, u[j, 0])
+    cl = native.GpuClosure(kind, b, epsilon=1e-9, safe_line_search=1, **extra)
+    nfail, lam, st, it = _run_gpu(cl, u, lam0)
+    lam_o = lam0.copy()
+    st_o, it_o = oracle.solve_closure(c, b.phi, b.phiw, u, lam_o)
+    assert (st_o == 0).all()
+    assert nfail == 0 and (st == 0).all()
+    assert np.abs(it - it_o).max() <= 1
+    scale = np.abs(lam_star).max(axis=1, keepdims=True)
+    assert (np.abs(lam - lam_o) / scale).max() < 1e-8
+    assert (np.abs(lam - lam_star) / scale).max() < 1e-7
+
+
+def test_adaptive_orders_and_host_entry(oracle):
+    # per-cell nTotal from nested total-degree levels (momentsolver.cpp:88)
+    b = native.Basis(2, [1, 2, 3, 4, 5, 6], 7)
+    ntl = b.ntotal_per_level
+    rng = np.random.default_rng(9)
+    n = 384
+    c = oracle.closure(oracle.EULER2D, eps=1e-9, safe=1)
+    nt = ntl[rng.integers(0, len(ntl), n)].astype(np.int32)
+    lam_star = np.zeros((n, b.M, 4))
+    for j in range(n):
+        rho, vx, vy, p = rng.uniform(0.5, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.5, 2)
+        lam_star[j, 0] = oracle.DS(c, np.array([rho, rho * vx, rho * vy, p / 0.4 + 0.5 * rho * (vx * vx + vy * vy)]))
+        lam_star[j, 1:nt[j]] = rng.normal(0, 0.01, (nt[j] - 1, 4)) * np.abs(lam_star[j, 0])
+    uq = oracle.eval_uq(c, b.phi, lam_star, nt_cell=nt)
+    u = oracle.reconstruct(4, b.phiw, np.ascontiguousarray(uq), nt_cell=nt)
+    lam0 = np.zeros_like(lam_star)
+    for j in range(n):
+        lam0[j, 0] = oracle.DS(c, u[j, 0])
+    print("case: adaptive Euler2D M<=28 Q=49", flush=True)
+    cl = native.GpuClosure(native.EULER2D, b, epsilon=1e-9, safe_line_search=1)
+    lam = lam0.copy()
+    st, it = cl.solve_host(u, lam, nt_cell=nt)
+    lam_o = lam0.copy()
+    st_o, it_o = oracle.solve_closure(c, b.phi, b.phiw, u, lam_o, nt_cell=nt)
+    assert (st == 0).all() and (st_o == 0).all()
+    scale = np.abs(lam_star).max(axis=1, keepdims=True)
+    assert (np.abs(lam - lam_o) / scale).max() < 1e-8
+    for j in range(n):
+        assert np.all(lam[j, nt[j]:] == 0.0)
+
+
+def test_empty_batch():
+    b = workloads.closure_basis()
+    cl = native.GpuClosure(native.EULER1D, b)
+    u = np.zeros((0, b.M, 3))
+    st, it = cl.solve_host(u, u.copy())
+    assert st.size == 0

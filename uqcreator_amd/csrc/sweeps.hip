@@ -1,0 +1,336 @@
+// sweeps.hip -- HBM-bound sweeps of the moment solver's step body:
+//   u(lambda) at the quadrature points   (momentsolver.cpp:99-101)
+//   moment reconstruction u = uQ phiw    (momentsolver.cpp:120-124)
+//   CFL time step, global min            (momentsolver.cpp:127-133)
+//   kinetic flux + explicit Euler update (expliciteuler.cpp:8-75) with the
+//     residual of the state-0 mean fused in (momentsolver.cpp:149-152)
+//   mean / variance of the solution      (momentsolver.cpp:302-328)
+#include <hip/hip_runtime.h>
+
+#include "uqk_device.h"
+#include "uqk_internal.h"
+
+namespace uqk {
+
+constexpr int kSweepThreads = 256;
+constexpr int kPartialBlocks = 1024;
+
+int sweep_partial_blocks(long) { return kPartialBlocks; }
+
+// One thread per (cell, quadrature point): uq(s,k) = U_mat(sum_i lambda(s,i) phi(k,i)).
+template <int KIND>
+__global__ void __launch_bounds__(kSweepThreads) eval_uq_kernel(SweepArgs a) {
+    constexpr int S = KindInfo<KIND>::S;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ncell * a.Q) return;
+    long cell = t / a.Q;
+    int k = (int)(t - cell * a.Q);
+    int nt = a.nt_cell ? a.nt_cell[cell] : a.M;
+    const double* lam = a.lam + cell * S * a.M;
+    double L[S], Uv[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) L[s] = 0.0;
+    for (int i = 0; i < nt; ++i) {
+        double p = a.phi[k * a.M + i];
+#pragma unroll
+        for (int s = 0; s < S; ++s) L[s] += lam[i * S + s] * p;
+    }
+    closure_U_mat<KIND>(a.c, L, Uv);
+    double* out = a.uq_out + t * S;
+#pragma unroll
+    for (int s = 0; s < S; ++s) out[s] = Uv[s];
+}
+
+// One thread per output moment (cell, i, s): u(s,i) = sum_k uq(s,k) phiw(k,i) for i < nt.
+__global__ void __launch_bounds__(kSweepThreads) reconstruct_kernel(SweepArgs a) {
+    const int S = a.S, M = a.M, Q = a.Q;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ncell * S * M) return;
+    long cell = t / (S * M);
+    int e = (int)(t - cell * S * M);
+    int i = e / S, s = e - i * S;
+    int nt = a.nt_cell ? a.nt_cell[cell] : M;
+    double acc = 0.0;
+    if (i < nt) {
+        const double* q = a.uq + cell * S * Q;
+        for (int k = 0; k < Q; ++k) acc += q[k * S + s] * a.phiw[k * M + i];
+    }
+    a.u_out[t] = acc;
+}
+
+__device__ __forceinline__ double block_min(double v) {
+    __shared__ double red[kSweepThreads / 64];
+    v = wave_min(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = fmin(r, red[w]);
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double red[kSweepThreads / 64];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r += red[w];
+    __syncthreads();
+    return r;
+}
+
+// Per-block minimum of ComputeDt over cells (grid-stride), then a one-block finish.
+template <int KIND>
+__global__ void __launch_bounds__(kSweepThreads) dt_partial_kernel(SweepArgs a) {
+    constexpr int S = KindInfo<KIND>::S;
+    double m = 1e10;
+    for (long cell = (long)blockIdx.x * blockDim.x + threadIdx.x; cell < a.ncell; cell += (long)gridDim.x * blockDim.x) {
+        const double* q = a.uq + cell * S * a.Q;
+        double dx = a.dx_param[cell];
+        for (int k = 0; k < a.Q; ++k) m = fmin(m, point_dt<S>(a.p, q + k * S, dx));
+    }
+    m = block_min(m);
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = m;
+}
+
+__global__ void __launch_bounds__(kSweepThreads) min_finish_kernel(SweepArgs a) {
+    double m = 1e10;
+    for (int t = threadIdx.x; t < a.npartial; t += blockDim.x) m = fmin(m, a.partial[t]);
+    m = block_min(m);
+    if (threadIdx.x == 0) *a.scalar_out = m;
+}
+
+__global__ void __launch_bounds__(kSweepThreads) sum_finish_kernel(SweepArgs a) {
+    double m = 0.0;
+    for (int t = threadIdx.x; t < a.npartial; t += blockDim.x) m += a.partial[t];
+    m = block_sum(m);
+    if (threadIdx.x == 0) *a.scalar_out = m;
+}
+
+// One wavefront per cell.  Lanes over quadrature points sum the point fluxes of
+// all edges (the projection onto phiw is linear, so the edges share one
+// projection); lanes over (i, s) then project and update.  The residual
+// |u_new(0,0) - u_old(0,0)| * area of each cell is accumulated per block.
+template <int KIND>
+__global__ void __launch_bounds__(kSweepThreads) advance_kernel(SweepArgs a) {
+    constexpr int S = KindInfo<KIND>::S;
+    extern __shared__ double smem[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int M = a.M, Q = a.Q, SM = S * M, E = a.nedge, D = a.dim;
+    double* Gs = smem + wave * Q * S;
+    const double dt = a.dt_dev ? *a.dt_dev : a.dt;
+    double res = 0.0;
+    const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
+    for (long cell = (long)blockIdx.x * (blockDim.x >> 6) + wave; cell < a.ncell; cell += nwaves) {
+        const int bc = a.bc[cell];
+        const double* us = a.u + cell * SM;  // reconstructed moments
+        double* un = a.u_out + cell * SM;    // in: previous moments, out: new moments
+        const double old00 = un[0];
+        if (bc == UQK_BC_DIRICHLET) {
+            for (int e = lane; e < SM; e += 64) un[e] = us[e];
+            if (lane == 0) res += fabs(us[0] - old00) * a.area[cell];
+            continue;
+        }
+        const int nt = a.nt_cell ? a.nt_cell[cell] : M;
+        const double* uqj = a.uq + cell * S * Q;
+        const bool wall = (bc == UQK_BC_NOSLIP || bc == UQK_BC_SWWALL);
+        for (int k = lane; k < Q; k += 64) {
+            double uj[S], gsum[S], G[S], gh[S], nbv[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                uj[s] = uqj[k * S + s];
+                gsum[s] = 0.0;
+            }
+            if constexpr (S >= 3) {
+                if (wall) {
+                    // ghost state: velocity reflected to the tangential part
+                    // (expliciteuler.cpp:25-58)
+                    double bx = a.bnormal[cell * D + 0], by = a.bnormal[cell * D + 1];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) gh[s] = uj[s];
+                    double vx = gh[1] / gh[0], vy = gh[2] / gh[0];
+                    double vn = bx * vx + by * vy;
+                    double tx = vx - vn * bx, ty = vy - vn * by;
+                    double rho = gh[0];
+                    if constexpr (S == 4) {
+                        if (bc == UQK_BC_NOSLIP) gh[3] += rho * 0.5 * ((tx * tx + ty * ty) - (vx * vx + vy * vy));
+                    }
+                    gh[1] = rho * tx;
+                    gh[2] = rho * ty;
+                }
+            }
+            for (int e = 0; e < E; ++e) {
+                const int nb = a.nbr[cell * E + e];
+                const double* nu = a.unit_normal + (cell * E + e) * D;
+                const double* nv = a.normal + (cell * E + e) * D;
+                double nux = nu[0], nuy = D > 1 ? nu[1] : 0.0, nx = nv[0], ny = D > 1 ? nv[1] : 0.0;
+                if (wall && nb == a.ncell) {
+                    point_flux<S>(a.p, gh, gh, nux, nuy, nx, ny, G);
+                } else {
+                    if (nb == a.ncell) {
+                        // the reference reads the never-written uQ[nCells] (zero)
+#pragma unroll
+                        for (int s = 0; s < S; ++s) nbv[s] = 0.0;
+                    } else {
+#pragma unroll
+                        for (int s = 0; s < S; ++s) nbv[s] = a.uq[((long)nb * Q + k) * S + s];
+                    }
+                    point_flux<S>(a.p, uj, nbv, nux, nuy, nx, ny, G);
+                }
+#pragma unroll
+                for (int s = 0; s < S; ++s) gsum[s] += G[s];
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) Gs[k * S + s] = gsum[s];
+        }
+        wave_sync();
+        const double f = dt / a.area[cell];
+        const bool fric = (S == 3 && a.p.problem == UQK_SHALLOW_WATER2D_PROBLEM && a.p.friction > 0.0);
+        for (int e = lane; e < SM; e += 64) {
+            int i = e / S, s = e - i * S;
+            double rhs = 0.0, src = 0.0;
+            if (i < nt) {
+                for (int k = 0; k < Q; ++k) rhs += Gs[k * S + s] * a.phiw[k * M + i];
+                if (fric && s > 0) {
+                    for (int k = 0; k < Q; ++k) {
+                        const double* q = uqj + k * S;
+                        double cf = a.p.friction * (1.0 + (a.p.friction_dim >= 0 ? a.xi[k * a.d + a.p.friction_dim] : 0.0));
+                        double sp = sqrt(q[1] * q[1] + q[2] * q[2]) / q[0];
+                        src += -cf * sp * q[s] / q[0] * a.phiw[k * M + i];
+                    }
+                }
+            }
+            double v = us[e] - f * rhs;
+            if (fric) v += dt * src;
+            un[e] = v;
+            if (e == 0) res += fabs(v - old00) * a.area[cell];
+        }
+        wave_sync();
+    }
+    __syncthreads();
+    res = block_sum(res);
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = res;
+}
+
+// One thread per cell: E and Var of each state from the closure at lambda.
+template <int KIND>
+__global__ void __launch_bounds__(kSweepThreads) mean_var_kernel(SweepArgs a) {
+    constexpr int S = KindInfo<KIND>::S;
+    long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= a.ncell) return;
+    int nt = a.nt_cell ? a.nt_cell[cell] : a.M;
+    const double* lam = a.lam + cell * S * a.M;
+    double E[S], V[S], L[S], Uv[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) E[s] = V[s] = 0.0;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int k = 0; k < a.Q; ++k) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) L[s] = 0.0;
+            for (int i = 0; i < nt; ++i) {
+                double p = a.phi[k * a.M + i];
+#pragma unroll
+                for (int s = 0; s < S; ++s) L[s] += lam[i * S + s] * p;
+            }
+            closure_U<KIND>(a.c, L, Uv);
+            double w0 = a.phiw[k * a.M + 0];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                if (pass == 0)
+                    E[s] += Uv[s] * w0;
+                else
+                    V[s] += (Uv[s] - E[s]) * (Uv[s] - E[s]) * w0;
+            }
+        }
+    }
+    double* out = a.mv_out + cell * 2 * S;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        out[s] = E[s];
+        out[S + s] = V[s];
+    }
+}
+
+__global__ void fill_u64_kernel(unsigned long long* p, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+#define UQK_DISPATCH(kind, FN, ...)                                                                     \
+    switch (kind) {                                                                                     \
+        case UQK_BOUNDED_BARRIER: FN<UQK_BOUNDED_BARRIER>(__VA_ARGS__); break;                         \
+        case UQK_EULER1D: FN<UQK_EULER1D>(__VA_ARGS__); break;                                         \
+        case UQK_EULER2D: FN<UQK_EULER2D>(__VA_ARGS__); break;                                         \
+        case UQK_SHALLOW_WATER2D: FN<UQK_SHALLOW_WATER2D>(__VA_ARGS__); break;                         \
+        case UQK_SHALLOW_WATER2D_BARRIER: FN<UQK_SHALLOW_WATER2D_BARRIER>(__VA_ARGS__); break;         \
+        default: return hipErrorInvalidValue;                                                           \
+    }
+
+template <int KIND> static void go_eval_uq(const SweepArgs& a, hipStream_t st) {
+    long total = a.ncell * a.Q;
+    int blocks = (int)((total + kSweepThreads - 1) / kSweepThreads);
+    hipLaunchKernelGGL(eval_uq_kernel<KIND>, dim3(blocks), dim3(kSweepThreads), 0, st, a);
+}
+hipError_t launch_eval_uq(int kind, const SweepArgs& a, hipStream_t st) {
+    if (a.ncell <= 0) return hipSuccess;
+    UQK_DISPATCH(kind, go_eval_uq, a, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_reconstruct(const SweepArgs& a, hipStream_t st) {
+    if (a.ncell <= 0) return hipSuccess;
+    long total = a.ncell * a.S * a.M;
+    int blocks = (int)((total + kSweepThreads - 1) / kSweepThreads);
+    hipLaunchKernelGGL(reconstruct_kernel, dim3(blocks), dim3(kSweepThreads), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int KIND> static void go_dt(const SweepArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(dt_partial_kernel<KIND>, dim3(kPartialBlocks), dim3(kSweepThreads), 0, st, a);
+}
+hipError_t launch_dt(int kind, const SweepArgs& a, hipStream_t st) {
+    UQK_DISPATCH(kind, go_dt, a, st);
+    SweepArgs b = a;
+    b.npartial = kPartialBlocks;
+    hipLaunchKernelGGL(min_finish_kernel, dim3(1), dim3(kSweepThreads), 0, st, b);
+    return hipGetLastError();
+}
+
+template <int KIND> static void go_advance(const SweepArgs& a, hipStream_t st) {
+    constexpr int S = KindInfo<KIND>::S;
+    size_t shmem = sizeof(double) * (kSweepThreads / 64) * a.Q * S;
+    hipLaunchKernelGGL(advance_kernel<KIND>, dim3(kPartialBlocks), dim3(kSweepThreads), shmem, st, a);
+}
+hipError_t launch_advance(int kind, const SweepArgs& a, hipStream_t st) {
+    UQK_DISPATCH(kind, go_advance, a, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(const SweepArgs& a, hipStream_t st) {
+    SweepArgs b = a;
+    b.npartial = kPartialBlocks;
+    hipLaunchKernelGGL(sum_finish_kernel, dim3(1), dim3(kSweepThreads), 0, st, b);
+    return hipGetLastError();
+}
+
+template <int KIND> static void go_mean_var(const SweepArgs& a, hipStream_t st) {
+    int blocks = (int)((a.ncell + kSweepThreads - 1) / kSweepThreads);
+    hipLaunchKernelGGL(mean_var_kernel<KIND>, dim3(blocks), dim3(kSweepThreads), 0, st, a);
+}
+hipError_t launch_mean_var(int kind, const SweepArgs& a, hipStream_t st) {
+    if (a.ncell <= 0) return hipSuccess;
+    UQK_DISPATCH(kind, go_mean_var, a, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long* p, int n, hipStream_t st) {
+    hipLaunchKernelGGL(fill_u64_kernel, dim3((n + 63) / 64), dim3(64), 0, st, p, n);
+    return hipGetLastError();
+}
+
+}  // namespace uqk

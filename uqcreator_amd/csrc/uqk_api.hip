@@ -1,0 +1,434 @@
+// uqk_api.hip -- the C ABI declared in include/uqk.h.
+//
+// A context is the GPU-side Closure object: it holds the basis tables in HBM,
+// the closure parameters (Closure::Create, closure.cpp:94-131), a stream and
+// the small device buffers the kernels reduce into.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "uqk.h"
+#include "uqk_internal.h"
+
+struct uqk_ctx {
+    int device = 0;
+    int kind = 0;
+    int S = 0, M = 0, Q = 0, d = 0;
+    int precision = UQK_FP64;
+    uqk::ClosureParams c{};
+    double* d_phi = nullptr;
+    double* d_phiw = nullptr;
+    double* d_xi = nullptr;
+    unsigned long long* d_counters = nullptr;  // [0] queue head, [1] failed cells, [2] Newton steps, [3] gradients
+    double* d_partial = nullptr;               // per-block partials
+    double* d_scalars = nullptr;               // [0] dt, [1] residual
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_closure_ms = 0.0;
+    unsigned long long last_stats[3] = {0, 0, 0};
+    size_t max_lds = 0;
+    int num_cu = 0;
+    std::string err;
+};
+
+namespace {
+
+int fail(uqk_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define UQK_HIP(ctx, call)                                                                        \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return fail(ctx, -2, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+uqk::SweepArgs sweep_args(const uqk_ctx* ctx) {
+    uqk::SweepArgs a{};
+    a.c = ctx->c;
+    a.S = ctx->S;
+    a.M = ctx->M;
+    a.Q = ctx->Q;
+    a.d = ctx->d;
+    a.phi = ctx->d_phi;
+    a.phiw = ctx->d_phiw;
+    a.xi = ctx->d_xi;
+    a.partial = ctx->d_partial;
+    a.npartial = uqk::sweep_partial_blocks(0);
+    return a;
+}
+
+uqk::ProblemParams problem_params(const uqk_problem* p) {
+    uqk::ProblemParams q{};
+    q.problem = p->problem;
+    q.flux = p->flux;
+    q.cfl = p->cfl;
+    q.gamma = p->gamma;
+    q.g = p->gravity;
+    q.friction = p->friction;
+    q.friction_dim = p->friction_dim;
+    return q;
+}
+
+void set_mesh(uqk::SweepArgs& a, const uqk_mesh* m) {
+    a.nedge = m->n_edges;
+    a.dim = m->dim;
+    a.nbr = m->neighbor;
+    a.unit_normal = m->unit_normal;
+    a.normal = m->normal;
+    a.area = m->area;
+    a.dx_param = m->dx_param;
+    a.bc = m->bc;
+    a.bnormal = m->boundary_normal;
+}
+
+int expected_states(int kind) {
+    switch (kind) {
+        case UQK_BOUNDED_BARRIER: return 1;
+        case UQK_EULER1D: return 3;
+        case UQK_EULER2D: return 4;
+        case UQK_SHALLOW_WATER2D: return 3;
+        case UQK_SHALLOW_WATER2D_BARRIER: return 3;
+    }
+    return -1;
+}
+
+// Launch the Newton closure kernel; leaves the failed-cell count in d_counters[1].
+int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, double* lam, int32_t* status,
+                   int32_t* iters) {
+    UQK_HIP(ctx, uqk::launch_fill_u64(ctx->d_counters, 4, ctx->stream));
+    if (n <= 0) return 0;
+    uqk::NewtonArgs a{};
+    a.c = ctx->c;
+    a.M = ctx->M;
+    a.Q = ctx->Q;
+    a.phi = ctx->d_phi;
+    a.phiw = ctx->d_phiw;
+    a.ncell = n;
+    a.nt_cell = nt;
+    a.u = u;
+    a.lam = lam;
+    a.status = status;
+    a.iters = iters;
+    a.counter = ctx->d_counters;
+    a.nfail = ctx->d_counters + 1;
+    a.wave_doubles = uqk::newton_fp64_wave_doubles(ctx->S, ctx->M, ctx->Q);
+    const size_t wave_bytes = sizeof(double) * (size_t)a.wave_doubles;
+    const size_t table_bytes = sizeof(double) * 2 * (size_t)ctx->Q * ctx->M;
+    bool tlds = false;
+    int waves = 0;
+    for (int w : {4, 2, 1}) {
+        if (table_bytes + w * wave_bytes <= ctx->max_lds) {
+            tlds = true;
+            waves = w;
+            break;
+        }
+    }
+    if (!tlds) {
+        for (int w : {4, 2, 1}) {
+            if (w * wave_bytes <= ctx->max_lds) {
+                waves = w;
+                break;
+            }
+        }
+    }
+    if (waves == 0)
+        return fail(ctx, -3, "closure too large for LDS: " + std::to_string(wave_bytes) + " bytes per cell");
+    const int threads = 64 * waves;
+    const size_t shmem = (tlds ? table_bytes : 0) + waves * wave_bytes;
+    int occ = uqk::newton_fp64_occupancy(ctx->kind, tlds, threads, shmem);
+    if (occ <= 0) occ = 1;
+    long want = (n + waves - 1) / waves;
+    int blocks = (int)std::min<long>(want, (long)occ * ctx->num_cu);
+    UQK_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    UQK_HIP(ctx, uqk::newton_fp64_launch(ctx->kind, a, tlds, blocks, threads, shmem, ctx->stream));
+    UQK_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    return 0;
+}
+
+int64_t finish_closure(uqk_ctx* ctx) {
+    UQK_HIP(ctx, hipMemcpyAsync(ctx->last_stats, ctx->d_counters + 1, sizeof(ctx->last_stats), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_closure_ms = ms;
+    return (int64_t)ctx->last_stats[0];
+}
+
+}  // namespace
+
+extern "C" {
+
+int uqk_create(const uqk_desc* desc, uqk_ctx** out) {
+    if (!desc || !out) return -1;
+    *out = nullptr;
+    uqk_ctx* ctx = new uqk_ctx();
+    int S = expected_states(desc->closure);
+    if (S < 0 || desc->n_states != S) {
+        int rc = fail(ctx, -1, "closure kind and n_states do not match");
+        *out = ctx;
+        return rc;
+    }
+    if (desc->n_moments <= 0 || desc->n_quad <= 0 || !desc->phi || !desc->phiw) {
+        int rc = fail(ctx, -1, "invalid sizes or tables");
+        *out = ctx;
+        return rc;
+    }
+    if (desc->precision != UQK_FP64 && desc->precision != UQK_MIXED) {
+        int rc = fail(ctx, -1, "unknown precision");
+        *out = ctx;
+        return rc;
+    }
+    *out = ctx;
+    ctx->device = desc->device;
+    ctx->kind = desc->closure;
+    ctx->S = S;
+    ctx->M = desc->n_moments;
+    ctx->Q = desc->n_quad;
+    ctx->d = desc->dim_xi;
+    ctx->precision = desc->precision;
+    ctx->c.gamma = desc->gamma;
+    ctx->c.u_minus = desc->u_minus;
+    ctx->c.u_plus = desc->u_plus;
+    ctx->c.g = desc->gravity;
+    ctx->c.barrier = desc->barrier;
+    ctx->c.alpha = desc->alpha;
+    ctx->c.eps = desc->epsilon;
+    ctx->c.max_iter = desc->max_iterations;
+    ctx->c.max_halvings = desc->max_halvings;
+    ctx->c.du_exact = desc->du_exact;
+    ctx->c.safe = desc->safe_line_search;
+    UQK_HIP(ctx, hipSetDevice(ctx->device));
+    hipDeviceProp_t prop;
+    UQK_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
+    ctx->max_lds = prop.sharedMemPerBlock;
+    ctx->num_cu = prop.multiProcessorCount;
+    const size_t tb = sizeof(double) * (size_t)ctx->Q * ctx->M;
+    UQK_HIP(ctx, hipMalloc(&ctx->d_phi, tb));
+    UQK_HIP(ctx, hipMalloc(&ctx->d_phiw, tb));
+    UQK_HIP(ctx, hipMemcpy(ctx->d_phi, desc->phi, tb, hipMemcpyHostToDevice));
+    UQK_HIP(ctx, hipMemcpy(ctx->d_phiw, desc->phiw, tb, hipMemcpyHostToDevice));
+    if (desc->xi && desc->dim_xi > 0) {
+        const size_t xb = sizeof(double) * (size_t)ctx->Q * desc->dim_xi;
+        UQK_HIP(ctx, hipMalloc(&ctx->d_xi, xb));
+        UQK_HIP(ctx, hipMemcpy(ctx->d_xi, desc->xi, xb, hipMemcpyHostToDevice));
+    }
+    UQK_HIP(ctx, hipMalloc(&ctx->d_counters, 4 * sizeof(unsigned long long)));
+    UQK_HIP(ctx, hipMalloc(&ctx->d_partial, sizeof(double) * uqk::sweep_partial_blocks(0)));
+    UQK_HIP(ctx, hipMalloc(&ctx->d_scalars, 4 * sizeof(double)));
+    UQK_HIP(ctx, hipStreamCreate(&ctx->own_stream));
+    ctx->stream = ctx->own_stream;
+    UQK_HIP(ctx, hipEventCreate(&ctx->ev0));
+    UQK_HIP(ctx, hipEventCreate(&ctx->ev1));
+    return 0;
+}
+
+int uqk_destroy(uqk_ctx* ctx) {
+    if (!ctx) return 0;
+    hipSetDevice(ctx->device);
+    if (ctx->own_stream) hipStreamSynchronize(ctx->own_stream);
+    hipFree(ctx->d_phi);
+    hipFree(ctx->d_phiw);
+    hipFree(ctx->d_xi);
+    hipFree(ctx->d_counters);
+    hipFree(ctx->d_partial);
+    hipFree(ctx->d_scalars);
+    if (ctx->ev0) hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) hipEventDestroy(ctx->ev1);
+    if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return 0;
+}
+
+const char* uqk_last_error(const uqk_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int uqk_set_stream(uqk_ctx* ctx, void* s) {
+    if (!ctx) return -1;
+    ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+    return 0;
+}
+
+int uqk_synchronize(uqk_ctx* ctx) {
+    if (!ctx) return -1;
+    UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int64_t uqk_closure_solve(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, double* lam,
+                          int32_t* status, int32_t* iters) {
+    if (!ctx) return -1;
+    UQK_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = launch_closure(ctx, n, nt, u, lam, status, iters);
+    if (rc) return rc;
+    return finish_closure(ctx);
+}
+
+int64_t uqk_closure_solve_host(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, double* lam,
+                               int32_t* status, int32_t* iters) {
+    if (!ctx) return -1;
+    UQK_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t cb = sizeof(double) * (size_t)n * ctx->S * ctx->M;
+    double *du = nullptr, *dl = nullptr;
+    int32_t *dnt = nullptr, *dst = nullptr, *dit = nullptr;
+    int64_t result = 0;
+    auto cleanup = [&]() {
+        hipFree(du);
+        hipFree(dl);
+        hipFree(dnt);
+        hipFree(dst);
+        hipFree(dit);
+    };
+#define UQK_HIP_C(call)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            cleanup();                                                                     \
+            return fail(ctx, -2, std::string(#call ": ") + hipGetErrorString(e_));         \
+        }                                                                                  \
+    } while (0)
+    if (n > 0) {
+        UQK_HIP_C(hipMalloc(&du, cb));
+        UQK_HIP_C(hipMalloc(&dl, cb));
+        UQK_HIP_C(hipMalloc(&dst, sizeof(int32_t) * n));
+        UQK_HIP_C(hipMalloc(&dit, sizeof(int32_t) * n));
+        UQK_HIP_C(hipMemcpyAsync(du, u, cb, hipMemcpyHostToDevice, ctx->stream));
+        UQK_HIP_C(hipMemcpyAsync(dl, lam, cb, hipMemcpyHostToDevice, ctx->stream));
+        if (nt) {
+            UQK_HIP_C(hipMalloc(&dnt, sizeof(int32_t) * n));
+            UQK_HIP_C(hipMemcpyAsync(dnt, nt, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+        }
+    }
+    int rc = launch_closure(ctx, n, dnt, du, dl, dst, dit);
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    result = finish_closure(ctx);
+    if (result >= 0 && n > 0) {
+        UQK_HIP_C(hipMemcpyAsync(lam, dl, cb, hipMemcpyDeviceToHost, ctx->stream));
+        if (status) UQK_HIP_C(hipMemcpyAsync(status, dst, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+        if (iters) UQK_HIP_C(hipMemcpyAsync(iters, dit, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+        UQK_HIP_C(hipStreamSynchronize(ctx->stream));
+    }
+#undef UQK_HIP_C
+    cleanup();
+    return result;
+}
+
+int uqk_eval_uq(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* lam, double* uq) {
+    if (!ctx) return -1;
+    uqk::SweepArgs a = sweep_args(ctx);
+    a.ncell = n;
+    a.nt_cell = nt;
+    a.lam = lam;
+    a.uq_out = uq;
+    UQK_HIP(ctx, uqk::launch_eval_uq(ctx->kind, a, ctx->stream));
+    UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int uqk_reconstruct(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* uq, double* u) {
+    if (!ctx) return -1;
+    uqk::SweepArgs a = sweep_args(ctx);
+    a.ncell = n;
+    a.nt_cell = nt;
+    a.uq = uq;
+    a.u_out = u;
+    UQK_HIP(ctx, uqk::launch_reconstruct(a, ctx->stream));
+    UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int uqk_compute_dt(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const double* uq, double* dt_host) {
+    if (!ctx || !prob || !mesh) return -1;
+    uqk::SweepArgs a = sweep_args(ctx);
+    a.p = problem_params(prob);
+    set_mesh(a, mesh);
+    a.ncell = mesh->n_cells;
+    a.uq = uq;
+    a.scalar_out = ctx->d_scalars;
+    UQK_HIP(ctx, uqk::launch_dt(ctx->kind, a, ctx->stream));
+    UQK_HIP(ctx, hipMemcpyAsync(dt_host, ctx->d_scalars, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int uqk_advance(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const int32_t* nt, double dt,
+                const double* u, const double* uq, double* unew) {
+    if (!ctx || !prob || !mesh) return -1;
+    uqk::SweepArgs a = sweep_args(ctx);
+    a.p = problem_params(prob);
+    set_mesh(a, mesh);
+    a.ncell = mesh->n_cells;
+    a.nt_cell = nt;
+    a.u = u;
+    a.uq = uq;
+    a.u_out = unew;
+    a.dt = dt;
+    UQK_HIP(ctx, uqk::launch_advance(ctx->kind, a, ctx->stream));
+    UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int64_t uqk_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const int32_t* nt, double* u,
+                 double* lam, double* uq, double* u_scratch, int32_t* status, int32_t* iters, double* dt_out,
+                 double* residual_out) {
+    if (!ctx || !prob || !mesh) return -1;
+    UQK_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t n = mesh->n_cells;
+    int rc = launch_closure(ctx, n, nt, u, lam, status, iters);
+    if (rc) return rc;
+    uqk::SweepArgs a = sweep_args(ctx);
+    a.p = problem_params(prob);
+    set_mesh(a, mesh);
+    a.ncell = n;
+    a.nt_cell = nt;
+    a.lam = lam;
+    a.uq_out = uq;
+    UQK_HIP(ctx, uqk::launch_eval_uq(ctx->kind, a, ctx->stream));
+    a.uq = uq;
+    a.u_out = u_scratch;
+    UQK_HIP(ctx, uqk::launch_reconstruct(a, ctx->stream));
+    a.scalar_out = ctx->d_scalars;
+    UQK_HIP(ctx, uqk::launch_dt(ctx->kind, a, ctx->stream));
+    a.u = u_scratch;
+    a.u_out = u;
+    a.dt_dev = ctx->d_scalars;
+    UQK_HIP(ctx, uqk::launch_advance(ctx->kind, a, ctx->stream));
+    a.scalar_out = ctx->d_scalars + 1;
+    UQK_HIP(ctx, uqk::launch_residual(a, ctx->stream));
+    double sc[2];
+    UQK_HIP(ctx, hipMemcpyAsync(sc, ctx->d_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    int64_t nfail = finish_closure(ctx);
+    if (dt_out) *dt_out = sc[0];
+    if (residual_out) *residual_out = sc[1];
+    return nfail;
+}
+
+int uqk_mean_var(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* lam, double* mv) {
+    if (!ctx) return -1;
+    uqk::SweepArgs a = sweep_args(ctx);
+    a.ncell = n;
+    a.nt_cell = nt;
+    a.lam = lam;
+    a.mv_out = mv;
+    UQK_HIP(ctx, uqk::launch_mean_var(ctx->kind, a, ctx->stream));
+    UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+double uqk_last_closure_ms(const uqk_ctx* ctx) { return ctx ? ctx->last_closure_ms : 0.0; }
+
+int uqk_last_closure_stats(const uqk_ctx* ctx, int64_t* newton_steps, int64_t* gradient_evaluations) {
+    if (!ctx) return -1;
+    if (newton_steps) *newton_steps = (int64_t)ctx->last_stats[1];
+    if (gradient_evaluations) *gradient_evaluations = (int64_t)ctx->last_stats[2];
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+// uqk_internal.h -- kernel argument blocks and launchers shared between the
+// kernel translation units and the C-ABI layer (uqk_api.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "uqk.h"
+#include "uqk_device.h"
+
+namespace uqk {
+
+struct NewtonArgs {
+    ClosureParams c;
+    int M;                       // table width (nTotal of the finest level)
+    int Q;                       // quadrature points
+    const double* phi;           // Q x M, device
+    const double* phiw;          // Q x M, device
+    long ncell;
+    const int* nt_cell;          // per-cell nTotal or nullptr
+    const double* u;             // ncell x S x M
+    double* lam;                 // ncell x S x M, in/out
+    int* status;                 // ncell or nullptr
+    int* iters;                  // ncell or nullptr
+    unsigned long long* counter; // work-queue head, zero at launch
+    unsigned long long* nfail;   // [0] failed cells, [1] Newton steps, [2] gradient evaluations; zero at launch
+    int wave_doubles;            // LDS doubles per wave
+};
+
+int newton_fp64_wave_doubles(int S, int M, int Q);
+int newton_fp64_occupancy(int kind, bool tlds, int threads, size_t shmem);
+hipError_t newton_fp64_launch(int kind, const NewtonArgs& a, bool tlds, int blocks, int threads, size_t shmem,
+                              hipStream_t st);
+
+struct SweepArgs {
+    ClosureParams c;
+    ProblemParams p;
+    int S, M, Q, d;
+    const double* phi;
+    const double* phiw;
+    const double* xi;            // Q x d
+    long ncell;
+    const int* nt_cell;
+    // mesh
+    int nedge, dim;
+    const int* nbr;
+    const double* unit_normal;
+    const double* normal;
+    const double* area;
+    const double* dx_param;
+    const int* bc;
+    const double* bnormal;
+    // data
+    const double* lam;
+    const double* u;
+    const double* uq;
+    double* uq_out;
+    double* u_out;
+    double* mv_out;
+    double dt;
+    const double* dt_dev;        // if non-null, dt is read from here
+    double* partial;             // per-block partial results
+    int npartial;
+    double* scalar_out;          // final reduction result
+};
+
+hipError_t launch_eval_uq(int kind, const SweepArgs& a, hipStream_t st);
+hipError_t launch_reconstruct(const SweepArgs& a, hipStream_t st);
+hipError_t launch_dt(int kind, const SweepArgs& a, hipStream_t st);
+hipError_t launch_advance(int kind, const SweepArgs& a, hipStream_t st);
+hipError_t launch_residual(const SweepArgs& a, hipStream_t st);
+hipError_t launch_mean_var(int kind, const SweepArgs& a, hipStream_t st);
+hipError_t launch_fill_u64(unsigned long long* p, int n, hipStream_t st);
+int sweep_partial_blocks(long ncell);
+
+}  // namespace uqk

@@ -1,0 +1,276 @@
+"""ctypes binding of the C ABI in include/uqk.h (libuqk.so).
+
+The library must have been built (``python -m uqcreator_amd.build`` or
+``__graft_entry__.build()``); loading fails loudly otherwise -- there is no
+CPU fallback for any entry point.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libuqk.so")
+
+BOUNDED_BARRIER, EULER1D, EULER2D, SHALLOW_WATER2D, SHALLOW_WATER2D_BARRIER = range(5)
+FP64, MIXED = 0, 1
+BURGERS, EULER2D_PROBLEM, SHALLOW_WATER2D_PROBLEM = range(3)
+FLUX_REFERENCE, FLUX_LAX_FRIEDRICHS = 0, 1
+BC_NONE, BC_DIRICHLET, BC_NOSLIP, BC_SWWALL = range(4)
+STATES = {BOUNDED_BARRIER: 1, EULER1D: 3, EULER2D: 4, SHALLOW_WATER2D: 3, SHALLOW_WATER2D_BARRIER: 3}
+# Newton damping per closure (boundedbarrier.cpp:4, eulerclosure.cpp:3,
+# eulerclosure2d.cpp:3, shallowwaterclosure2d.cpp:3-6)
+ALPHA = {BOUNDED_BARRIER: 1.0, EULER1D: 0.5, EULER2D: 1.0, SHALLOW_WATER2D: 1.0, SHALLOW_WATER2D_BARRIER: 1.0}
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("closure", C.c_int32), ("n_states", C.c_int32), ("n_moments", C.c_int32), ("n_quad", C.c_int32),
+        ("dim_xi", C.c_int32), ("gamma", C.c_double), ("u_minus", C.c_double), ("u_plus", C.c_double),
+        ("gravity", C.c_double), ("barrier", C.c_double), ("alpha", C.c_double), ("epsilon", C.c_double),
+        ("max_iterations", C.c_int32), ("max_halvings", C.c_int32), ("du_exact", C.c_int32),
+        ("safe_line_search", C.c_int32), ("precision", C.c_int32), ("phi", C.c_void_p), ("phiw", C.c_void_p),
+        ("xi", C.c_void_p), ("device", C.c_int32),
+    ]
+
+
+class ProblemDesc(C.Structure):
+    _fields_ = [
+        ("problem", C.c_int32), ("flux", C.c_int32), ("cfl", C.c_double), ("gamma", C.c_double),
+        ("gravity", C.c_double), ("friction", C.c_double), ("friction_dim", C.c_int32),
+    ]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("n_cells", C.c_int64), ("n_edges", C.c_int32), ("dim", C.c_int32), ("neighbor", C.c_void_p),
+        ("unit_normal", C.c_void_p), ("normal", C.c_void_p), ("area", C.c_void_p), ("dx_param", C.c_void_p),
+        ("bc", C.c_void_p), ("boundary_normal", C.c_void_p),
+    ]
+
+
+# Every symbol declared in include/uqk.h, with its ctypes signature.
+SIGNATURES = {
+    "uqk_create": (C.c_int, [C.POINTER(Desc), C.POINTER(C.c_void_p)]),
+    "uqk_destroy": (C.c_int, [C.c_void_p]),
+    "uqk_last_error": (C.c_char_p, [C.c_void_p]),
+    "uqk_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "uqk_synchronize": (C.c_int, [C.c_void_p]),
+    "uqk_closure_solve": (C.c_int64, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
+    "uqk_closure_solve_host": (C.c_int64, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p]),
+    "uqk_eval_uq": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uqk_reconstruct": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uqk_compute_dt": (C.c_int, [C.c_void_p, C.POINTER(ProblemDesc), C.POINTER(MeshDesc), C.c_void_p,
+                                 C.POINTER(C.c_double)]),
+    "uqk_advance": (C.c_int, [C.c_void_p, C.POINTER(ProblemDesc), C.POINTER(MeshDesc), C.c_void_p, C.c_double,
+                              C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uqk_step": (C.c_int64, [C.c_void_p, C.POINTER(ProblemDesc), C.POINTER(MeshDesc), C.c_void_p, C.c_void_p,
+                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
+                             C.POINTER(C.c_double)]),
+    "uqk_mean_var": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uqk_gauss_legendre": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p]),
+    "uqk_index_set": (C.c_int32, [C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
+    "uqk_basis_tables": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p]),
+    "uqk_last_closure_ms": (C.c_double, [C.c_void_p]),
+    "uqk_last_closure_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libuqk.so; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libuqk.so is not built (%s); run python -m uqcreator_amd.build" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(C.c_void_p)
+    return C.c_void_p(a.data_ptr())  # torch tensor
+
+
+# ---------------------------------------------------------------------------
+# host numerics (Settings index set, quadrature, Closure tables)
+# ---------------------------------------------------------------------------
+def gauss_legendre(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    if lib().uqk_gauss_legendre(n, _p(x), _p(w)) != 0:
+        raise RuntimeError("gauss_legendre failed")
+    return x, w
+
+
+def index_set(d, degrees, max_degree=False):
+    deg = np.asarray(degrees, dtype=np.int32)
+    cap = (int(deg[-1]) + 1) ** d
+    idx = np.zeros(cap * d, dtype=np.int32)
+    ntl = np.zeros(len(deg), dtype=np.int32)
+    n = lib().uqk_index_set(d, len(deg), _p(deg), int(max_degree), cap, _p(idx), _p(ntl))
+    if n < 0:
+        raise RuntimeError("index_set failed")
+    return idx[: n * d].reshape(n, d), ntl
+
+
+def basis_tables(d, q1, idx):
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    M = idx.shape[0]
+    Q = q1 ** d
+    xi = np.zeros((Q, d))
+    wq = np.zeros(Q)
+    phi = np.zeros((Q, M))
+    phiw = np.zeros((Q, M))
+    r = lib().uqk_basis_tables(d, q1, M, _p(idx), _p(xi), _p(wq), _p(phi), _p(phiw))
+    if r != Q:
+        raise RuntimeError("basis_tables failed")
+    return xi, wq, phi, phiw
+
+
+class Basis:
+    """Index set, quadrature and the Closure's phi / phiw tables."""
+
+    def __init__(self, d, degrees, q1, max_degree=False):
+        self.d = d
+        self.degrees = list(degrees)
+        self.q1 = q1
+        self.idx, self.ntotal_per_level = index_set(d, degrees, max_degree)
+        self.xi, self.wq, self.phi, self.phiw = basis_tables(d, q1, self.idx)
+        self.Q, self.M = self.phi.shape
+
+
+# ---------------------------------------------------------------------------
+# GPU closure context
+# ---------------------------------------------------------------------------
+class GpuClosure:
+    """The GPU Closure: uqk_create / uqk_closure_solve / sweeps, one context."""
+
+    def __init__(self, kind, basis, gamma=1.4, u_minus=3.0, u_plus=12.0, gravity=9.81, barrier=0.0, alpha=None,
+                 epsilon=1e-10, max_iterations=1000, max_halvings=1000, du_exact=1, safe_line_search=0,
+                 precision=FP64, device=0):
+        self.kind = kind
+        self.S = STATES[kind]
+        self.basis = basis
+        self.M, self.Q = basis.M, basis.Q
+        self._phi = np.ascontiguousarray(basis.phi)
+        self._phiw = np.ascontiguousarray(basis.phiw)
+        self._xi = np.ascontiguousarray(basis.xi)
+        self.desc = Desc(kind, self.S, self.M, self.Q, basis.d, gamma, u_minus, u_plus, gravity, barrier,
+                         ALPHA[kind] if alpha is None else alpha, epsilon, max_iterations, max_halvings, du_exact,
+                         safe_line_search, precision, _p(self._phi), _p(self._phiw), _p(self._xi), device)
+        h = C.c_void_p()
+        rc = lib().uqk_create(C.byref(self.desc), C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = lib().uqk_last_error(h).decode() if h else "uqk_create failed"
+            self.close()
+            raise RuntimeError("uqk_create: %s (%d)" % (msg, rc))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().uqk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise RuntimeError("%s: %s (%d)" % (what, lib().uqk_last_error(self.h).decode(), rc))
+        return rc
+
+    def set_stream(self, stream_handle):
+        self._check(lib().uqk_set_stream(self.h, C.c_void_p(stream_handle)), "uqk_set_stream")
+
+    def solve(self, u, lam, status=None, iters=None, nt_cell=None):
+        """Device tensors (ncell, M, S) float64; lam updated in place. Returns #failed cells."""
+        n = u.shape[0]
+        return self._check(lib().uqk_closure_solve(self.h, n, _p(nt_cell), _p(u), _p(lam), _p(status), _p(iters)),
+                           "uqk_closure_solve")
+
+    def solve_host(self, u, lam, nt_cell=None):
+        """numpy (ncell, M, S) float64; lam updated in place. Returns (status, iters)."""
+        n = u.shape[0]
+        assert u.flags.c_contiguous and lam.flags.c_contiguous
+        status = np.zeros(n, dtype=np.int32)
+        iters = np.zeros(n, dtype=np.int32)
+        ntc = None if nt_cell is None else np.ascontiguousarray(nt_cell, dtype=np.int32)
+        self._check(lib().uqk_closure_solve_host(self.h, n, _p(ntc), _p(u), _p(lam), _p(status), _p(iters)),
+                    "uqk_closure_solve_host")
+        return status, iters
+
+    def eval_uq(self, lam, uq, nt_cell=None):
+        self._check(lib().uqk_eval_uq(self.h, lam.shape[0], _p(nt_cell), _p(lam), _p(uq)), "uqk_eval_uq")
+
+    def reconstruct(self, uq, u, nt_cell=None):
+        self._check(lib().uqk_reconstruct(self.h, uq.shape[0], _p(nt_cell), _p(uq), _p(u)), "uqk_reconstruct")
+
+    def compute_dt(self, prob, mesh, uq):
+        dt = C.c_double()
+        self._check(lib().uqk_compute_dt(self.h, C.byref(prob), C.byref(mesh.desc), _p(uq), C.byref(dt)),
+                    "uqk_compute_dt")
+        return dt.value
+
+    def advance(self, prob, mesh, dt, u, uq, unew, nt_cell=None):
+        self._check(lib().uqk_advance(self.h, C.byref(prob), C.byref(mesh.desc), _p(nt_cell), C.c_double(dt), _p(u),
+                                      _p(uq), _p(unew)), "uqk_advance")
+
+    def step(self, prob, mesh, u, lam, uq, u_scratch, status=None, iters=None, nt_cell=None):
+        dt = C.c_double()
+        res = C.c_double()
+        nfail = self._check(lib().uqk_step(self.h, C.byref(prob), C.byref(mesh.desc), _p(nt_cell), _p(u), _p(lam),
+                                           _p(uq), _p(u_scratch), _p(status), _p(iters), C.byref(dt), C.byref(res)),
+                            "uqk_step")
+        return nfail, dt.value, res.value
+
+    def mean_var(self, lam, mv, nt_cell=None):
+        self._check(lib().uqk_mean_var(self.h, lam.shape[0], _p(nt_cell), _p(lam), _p(mv)), "uqk_mean_var")
+
+    def last_closure_ms(self):
+        return lib().uqk_last_closure_ms(self.h)
+
+    def last_closure_stats(self):
+        a, b = C.c_int64(), C.c_int64()
+        lib().uqk_last_closure_stats(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+
+def problem(kind, flux=FLUX_REFERENCE, cfl=0.5, gamma=1.4, gravity=9.81, friction=0.0, friction_dim=-1):
+    return ProblemDesc(kind, flux, cfl, gamma, gravity, friction, friction_dim)
+
+
+class DeviceMesh:
+    """Mesh arrays uploaded to the device as torch tensors (kept alive here)."""
+
+    def __init__(self, mesh, device="cuda"):
+        import torch
+
+        def t(a, dt):
+            return torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
+
+        self.nbr = t(mesh.nbr, torch.int32)
+        self.unit_normal = t(mesh.unit_normal, torch.float64)
+        self.normal = t(mesh.normal, torch.float64)
+        self.area = t(mesh.area, torch.float64)
+        self.dx_param = t(mesh.dx_param, torch.float64)
+        self.bc = t(mesh.bc, torch.int32)
+        self.bnormal = t(mesh.bnormal, torch.float64)
+        ncell, nedge = mesh.nbr.shape
+        self.desc = MeshDesc(ncell, nedge, mesh.unit_normal.shape[-1], _p(self.nbr), _p(self.unit_normal),
+                             _p(self.normal), _p(self.area), _p(self.dx_param), _p(self.bc), _p(self.bnormal))
