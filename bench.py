@@ -56,47 +56,46 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
 
     from uqcreator_amd import native, workloads
 
-    dev = "cuda:0"
-    torch.cuda.set_device(0)
     b = workloads.closure_basis()
     S, M, Q = 3, b.M, b.Q
     t0 = time.time()
-    u, lam_star, lam0 = workloads.closure_euler1d(args.ncell, basis=b, device=dev)
-    torch.cuda.synchronize()
+    u, lam_star, lam0 = workloads.closure_euler1d(args.ncell, basis=b, device=True)
+    native.synchronize()
     gen_s = time.time() - t0
     prec = native.FP64 if args.precision == "fp64" else native.MIXED
     cl = native.GpuClosure(native.EULER1D, b, epsilon=1e-10, du_exact=1, safe_line_search=1, precision=prec)
-    cl.set_stream(torch.cuda.current_stream().cuda_stream)
-    lam = torch.empty_like(lam0)
-    st = torch.zeros(args.ncell, dtype=torch.int32, device=dev)
-    it = torch.zeros(args.ncell, dtype=torch.int32, device=dev)
+    lam = native.DeviceArray(lam0.shape)
+    st = native.DeviceArray(args.ncell, np.int32)
+    it = native.DeviceArray(args.ncell, np.int32)
 
     for _ in range(args.warmup):
-        lam.copy_(lam0)
+        lam.copy_from(lam0)
         cl.solve(u, lam, st, it)
-    torch.cuda.synchronize()
+    native.synchronize()
     kernel_ms, newton, grads = [], 0, 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        lam.copy_(lam0)
-        nfail = cl.solve(u, lam, st, it)
+        lam.copy_from(lam0)
+        native.synchronize()
+        cl.solve(u, lam, st, it)
         kernel_ms.append(cl.last_closure_ms())
         a, g = cl.last_closure_stats()
         newton += a
         grads += g
-    torch.cuda.synchronize()
+    native.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
     value = args.ncell / (elapsed / args.steps)
 
-    status = st.cpu().numpy()
-    iters = it.cpu().numpy()
-    scale = lam_star.abs().amax(dim=1, keepdim=True)
-    lam_err = ((lam - lam_star).abs() / scale).amax().item()
+    status = st.download()
+    iters = it.download()
+    lam_h = lam.download()
+    lam_star_h = lam_star.download()
+    scale = np.abs(lam_star_h).max(axis=1, keepdims=True)
+    lam_err = float((np.abs(lam_h - lam_star_h) / scale).max())
 
     fm = flop_model(S, M, Q)
     newton_per_launch = newton / args.steps
@@ -149,14 +148,14 @@ def main():
         import oracle_py
 
         ns = min(args.cpu_sample, args.ncell)
-        us = np.ascontiguousarray(u[:ns].cpu().numpy())
-        ls = np.ascontiguousarray(lam0[:ns].cpu().numpy())
+        us = np.ascontiguousarray(u.download()[:ns])
+        ls = np.ascontiguousarray(lam0.download()[:ns])
         nth = cpu_threads()
         c = oracle_py.closure(oracle_py.EULER1D, eps=1e-10, du_exact=1, safe=1)
         t0 = time.perf_counter()
         st_o, it_o = oracle_py.solve_closure(c, b.phi, b.phiw, us, ls, nthreads=nth)
         cpu_s = time.perf_counter() - t0
-        lam_gpu = lam[:ns].cpu().numpy()
+        lam_gpu = lam_h[:ns]
         out["cpu_baseline"] = {
             "value": ns / cpu_s,
             "unit": "cell-steps/s",

@@ -164,6 +164,26 @@ int32_t uqk_index_set(int32_t d, int32_t n_levels, const int32_t* degrees, int32
 int32_t uqk_basis_tables(int32_t d, int32_t q1, int32_t m, const int32_t* idx, double* xi, double* wq, double* phi,
                          double* phiw);
 
+/* ---- synthetic workload and device memory ------------------------------ */
+/* Batched-closure benchmark input (no reference counterpart): 1D Euler cells
+ * with seeded random duals lambda* around a Maxwellian (counter-based Philox,
+ * key = seed, counter = (cell, attempt, draw)), moments u = <U(lambda* phi) phi>
+ * and the Newton start lambda0 = (DS(u_0), 0, ...).  phi/phiw are host tables
+ * (Q x M); u, lam_star, lam0 are (ncell x M x 3).  The first form writes device
+ * buffers, the _host form host buffers.  Returns the number of cells that
+ * found no admissible draw (0 expected), or a negative error. */
+int64_t uqk_synth_euler1d_closure(int64_t ncell, uint64_t seed, int32_t M, int32_t Q, const double* phi,
+                                  const double* phiw, double* u, double* lam_star, double* lam0);
+int64_t uqk_synth_euler1d_closure_host(int64_t ncell, uint64_t seed, int32_t M, int32_t Q, const double* phi,
+                                       const double* phiw, double* u, double* lam_star, double* lam0);
+/* Device memory on the current device, for callers without an allocator.
+ * uqk_memcpy kind: 0 host->device, 1 device->host, 2 device->device. */
+void* uqk_device_alloc(int64_t bytes);
+int uqk_device_free(void* ptr);
+int uqk_memcpy(void* dst, const void* src, int64_t bytes, int32_t kind);
+int uqk_device_synchronize(void);
+int32_t uqk_device_count(void);
+
 /* Milliseconds spent in the closure kernel by the last uqk_closure_solve /
  * uqk_step call, timed with HIP events on the launch stream. */
 double uqk_last_closure_ms(const uqk_ctx* ctx);

@@ -7,27 +7,20 @@ from uqcreator_amd import native, workloads
 pytestmark = pytest.mark.gpu
 
 
-def _dev(a):
-    import torch
-
-    return torch.as_tensor(np.ascontiguousarray(a), device="cuda")
-
-
 def _run_gpu(cl, u, lam0, nt=None):
-    import torch
-
     n = u.shape[0]
-    du, dl = _dev(u), _dev(lam0)
-    st = torch.zeros(n, dtype=torch.int32, device="cuda")
-    it = torch.zeros(n, dtype=torch.int32, device="cuda")
-    dnt = None if nt is None else _dev(np.asarray(nt, dtype=np.int32))
+    du = native.DeviceArray.from_host(u)
+    dl = native.DeviceArray.from_host(lam0)
+    st = native.DeviceArray(n, np.int32)
+    it = native.DeviceArray(n, np.int32)
+    dnt = None if nt is None else native.DeviceArray.from_host(np.asarray(nt, dtype=np.int32))
     nfail = cl.solve(du, dl, st, it, nt_cell=dnt)
-    return nfail, dl.cpu().numpy(), st.cpu().numpy(), it.cpu().numpy()
+    return nfail, dl.download(), st.download(), it.download()
 
 
 def test_euler1d_closure_matches_oracle(oracle):
     b = workloads.closure_basis()
-    u, lam_star, lam0 = (t.numpy() for t in workloads.closure_euler1d(4096, basis=b, seed=11))
+    u, lam_star, lam0 = workloads.closure_euler1d(4096, basis=b, seed=11)
     print("case: Euler1D S=3 M=15 Q=49, 4096 cells", flush=True)
     cl = native.GpuClosure(native.EULER1D, b, epsilon=1e-10, du_exact=1, safe_line_search=1)
     nfail, lam, st, it = _run_gpu(cl, u, lam0)
@@ -46,7 +39,7 @@ def test_euler1d_closure_matches_oracle(oracle):
 
 def test_euler1d_unsafe_and_one_shot(oracle):
     b = workloads.closure_basis()
-    u, lam_star, lam0 = (t.numpy() for t in workloads.closure_euler1d(512, basis=b, seed=5))
+    u, lam_star, lam0 = workloads.closure_euler1d(512, basis=b, seed=5)
     for kw, okw in [(dict(safe_line_search=0), dict(safe=0)), (dict(max_iterations=1), dict(max_iter=1)),
                     (dict(du_exact=0, safe_line_search=1), dict(du_exact=0, safe=1))]:
         print("case:", kw, flush=True)

@@ -139,8 +139,9 @@ def _c2_case(oracle, n=64, seed=3):
         pass
 
     b = B()
-    b.phi, b.phiw, b.M = phi, phiw, phi.shape[1]
-    u, lam_star, lam0 = (t.numpy() for t in workloads.closure_euler1d(n, basis=b, seed=seed))
+    b.phi, b.phiw = phi, phiw
+    b.Q, b.M = phi.shape
+    u, lam_star, lam0 = workloads.closure_euler1d(n, basis=b, seed=seed)
     return phi, phiw, u, lam_star, lam0
 
 

@@ -76,6 +76,15 @@ SIGNATURES = {
                                      C.c_void_p, C.c_void_p]),
     "uqk_last_closure_ms": (C.c_double, [C.c_void_p]),
     "uqk_last_closure_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "uqk_synth_euler1d_closure": (C.c_int64, [C.c_int64, C.c_uint64, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uqk_synth_euler1d_closure_host": (C.c_int64, [C.c_int64, C.c_uint64, C.c_int32, C.c_int32, C.c_void_p,
+                                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uqk_device_alloc": (C.c_void_p, [C.c_int64]),
+    "uqk_device_free": (C.c_int, [C.c_void_p]),
+    "uqk_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32]),
+    "uqk_device_synchronize": (C.c_int, []),
+    "uqk_device_count": (C.c_int32, []),
 }
 
 _lib = None
@@ -101,7 +110,66 @@ def _p(a):
         return None
     if isinstance(a, np.ndarray):
         return a.ctypes.data_as(C.c_void_p)
-    return C.c_void_p(a.data_ptr())  # torch tensor
+    if isinstance(a, DeviceArray):
+        return C.c_void_p(a.ptr)
+    return C.c_void_p(a.data_ptr())  # any object exposing a device pointer
+
+
+class DeviceArray:
+    """A device buffer owned through the C ABI (uqk_device_alloc), with the
+    shape/dtype of the host array it mirrors."""
+
+    def __init__(self, shape, dtype=np.float64):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        self.ptr = lib().uqk_device_alloc(max(self.nbytes, 1))
+        if not self.ptr:
+            raise MemoryError("uqk_device_alloc(%d) failed" % self.nbytes)
+
+    @classmethod
+    def from_host(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        if lib().uqk_memcpy(C.c_void_p(self.ptr), _p(a), self.nbytes, 0) != 0:
+            raise RuntimeError("host->device copy failed")
+
+    def download(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        if lib().uqk_memcpy(_p(out), C.c_void_p(self.ptr), self.nbytes, 1) != 0:
+            raise RuntimeError("device->host copy failed")
+        return out
+
+    def copy_from(self, other):
+        assert other.nbytes == self.nbytes
+        if lib().uqk_memcpy(C.c_void_p(self.ptr), C.c_void_p(other.ptr), self.nbytes, 2) != 0:
+            raise RuntimeError("device->device copy failed")
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().uqk_device_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_count():
+    return lib().uqk_device_count()
+
+
+def synchronize():
+    if lib().uqk_device_synchronize() != 0:
+        raise RuntimeError("device synchronize failed")
 
 
 # ---------------------------------------------------------------------------
@@ -256,21 +324,33 @@ def problem(kind, flux=FLUX_REFERENCE, cfl=0.5, gamma=1.4, gravity=9.81, frictio
 
 
 class DeviceMesh:
-    """Mesh arrays uploaded to the device as torch tensors (kept alive here)."""
+    """Mesh arrays uploaded to device buffers (kept alive here)."""
 
-    def __init__(self, mesh, device="cuda"):
-        import torch
-
-        def t(a, dt):
-            return torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
-
-        self.nbr = t(mesh.nbr, torch.int32)
-        self.unit_normal = t(mesh.unit_normal, torch.float64)
-        self.normal = t(mesh.normal, torch.float64)
-        self.area = t(mesh.area, torch.float64)
-        self.dx_param = t(mesh.dx_param, torch.float64)
-        self.bc = t(mesh.bc, torch.int32)
-        self.bnormal = t(mesh.bnormal, torch.float64)
+    def __init__(self, mesh):
+        self.nbr = DeviceArray.from_host(mesh.nbr, np.int32)
+        self.unit_normal = DeviceArray.from_host(mesh.unit_normal, np.float64)
+        self.normal = DeviceArray.from_host(mesh.normal, np.float64)
+        self.area = DeviceArray.from_host(mesh.area, np.float64)
+        self.dx_param = DeviceArray.from_host(mesh.dx_param, np.float64)
+        self.bc = DeviceArray.from_host(mesh.bc, np.int32)
+        self.bnormal = DeviceArray.from_host(mesh.bnormal, np.float64)
         ncell, nedge = mesh.nbr.shape
         self.desc = MeshDesc(ncell, nedge, mesh.unit_normal.shape[-1], _p(self.nbr), _p(self.unit_normal),
                              _p(self.normal), _p(self.area), _p(self.dx_param), _p(self.bc), _p(self.bnormal))
+
+
+def synth_closure_euler1d(ncell, basis, seed=20261202, device=True):
+    """The batched-closure workload (see include/uqk.h).  Returns (u, lam_star,
+    lam0) as DeviceArrays (device=True) or numpy arrays, each (ncell, M, 3)."""
+    M, Q = basis.M, basis.Q
+    phi = np.ascontiguousarray(basis.phi)
+    phiw = np.ascontiguousarray(basis.phiw)
+    if device:
+        out = [DeviceArray((ncell, M, 3)) for _ in range(3)]
+        nf = lib().uqk_synth_euler1d_closure(ncell, seed, M, Q, _p(phi), _p(phiw), *[_p(a) for a in out])
+    else:
+        out = [np.zeros((ncell, M, 3)) for _ in range(3)]
+        nf = lib().uqk_synth_euler1d_closure_host(ncell, seed, M, Q, _p(phi), _p(phiw), *[_p(a) for a in out])
+    if nf != 0:
+        raise RuntimeError("workload generation failed (%d)" % nf)
+    return tuple(out)
