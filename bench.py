@@ -49,6 +49,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--cpu-sample", type=int, default=1 << 14)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel", choices=["auto", "generic", "wave"], default="auto")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the closure is independent per cell and this build measures one "
@@ -67,6 +68,7 @@ def main():
     gen_s = time.time() - t0
     prec = native.FP64 if args.precision == "fp64" else native.MIXED
     cl = native.GpuClosure(native.EULER1D, b, epsilon=1e-10, du_exact=1, safe_line_search=1, precision=prec)
+    cl.set_kernel({"auto": native.KERNEL_AUTO, "generic": native.KERNEL_GENERIC, "wave": native.KERNEL_WAVE}[args.kernel])
     lam = native.DeviceArray(lam0.shape)
     st = native.DeviceArray(args.ncell, np.int32)
     it = native.DeviceArray(args.ncell, np.int32)
@@ -130,7 +132,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": None,
-            "kernel": "newton_fp64_kernel",
+            "kernel": "newton_wave_kernel" if cl.last_kernel() == native.KERNEL_WAVE else "newton_fp64_kernel",
             "kernel_ms": kms,
             "flops_per_launch": flops,
             "flop_model": fm,

@@ -116,6 +116,14 @@ const char* uqk_last_error(const uqk_ctx* ctx);
 int uqk_set_stream(uqk_ctx* ctx, void* hip_stream);
 int uqk_synchronize(uqk_ctx* ctx);
 
+/* Closure kernel selection: AUTO picks the compile-time specialised wave
+ * kernel when one exists for (closure, M, Q) and the batch has one order,
+ * else the generic kernel (any size, per-cell orders). */
+enum { UQK_KERNEL_AUTO = 0, UQK_KERNEL_GENERIC = 1, UQK_KERNEL_WAVE = 2 };
+int uqk_set_closure_kernel(uqk_ctx* ctx, int32_t kernel);
+/* Kernel used by the last closure call (UQK_KERNEL_GENERIC / UQK_KERNEL_WAVE). */
+int32_t uqk_last_closure_kernel(const uqk_ctx* ctx);
+
 /* Closure::SolveClosure / SolveClosureSafe (closure.cpp:133-193, :297-339),
  * batched over cells: the per-cell loop of momentsolver.cpp:83-89.
  * lambda: in = warm start, out = solution.  ntotal_cell: per-cell nTotal

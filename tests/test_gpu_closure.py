@@ -18,12 +18,18 @@ def _run_gpu(cl, u, lam0, nt=None):
     return nfail, dl.download(), st.download(), it.download()
 
 
-def test_euler1d_closure_matches_oracle(oracle):
+KERNELS = [native.KERNEL_GENERIC, native.KERNEL_WAVE]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_euler1d_closure_matches_oracle(oracle, kernel):
     b = workloads.closure_basis()
     u, lam_star, lam0 = workloads.closure_euler1d(4096, basis=b, seed=11)
-    print("case: Euler1D S=3 M=15 Q=49, 4096 cells", flush=True)
+    print("case: Euler1D S=3 M=15 Q=49, 4096 cells, kernel", kernel, flush=True)
     cl = native.GpuClosure(native.EULER1D, b, epsilon=1e-10, du_exact=1, safe_line_search=1)
+    cl.set_kernel(kernel)
     nfail, lam, st, it = _run_gpu(cl, u, lam0)
+    assert cl.last_kernel() == kernel
     c = oracle.closure(oracle.EULER1D, eps=1e-10, du_exact=1, safe=1)
     lam_o = lam0.copy()
     st_o, it_o = oracle.solve_closure(c, b.phi, b.phiw, u, lam_o)
@@ -37,14 +43,17 @@ def test_euler1d_closure_matches_oracle(oracle):
         assert np.linalg.norm(oracle.gradient(c, b.phi, b.phiw, lam[j], u[j])) < 1e-10
 
 
-def test_euler1d_unsafe_and_one_shot(oracle):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_euler1d_unsafe_and_one_shot(oracle, kernel):
     b = workloads.closure_basis()
     u, lam_star, lam0 = workloads.closure_euler1d(512, basis=b, seed=5)
     for kw, okw in [(dict(safe_line_search=0), dict(safe=0)), (dict(max_iterations=1), dict(max_iter=1)),
                     (dict(du_exact=0, safe_line_search=1), dict(du_exact=0, safe=1))]:
-        print("case:", kw, flush=True)
+        print("case:", kw, "kernel", kernel, flush=True)
         cl = native.GpuClosure(native.EULER1D, b, **{"du_exact": 1, **kw})
+        cl.set_kernel(kernel)
         nfail, lam, st, it = _run_gpu(cl, u, lam0)
+        assert cl.last_kernel() == kernel
         c = oracle.closure(oracle.EULER1D, **{"du_exact": 1, **okw})
         lam_o = lam0.copy()
         st_o, it_o = oracle.solve_closure(c, b.phi, b.phiw, u, lam_o)
@@ -72,13 +81,14 @@ def _moments(oracle, c, b, lam):
     return oracle.reconstruct(c.S, b.phiw, np.ascontiguousarray(uq))
 
 
+@pytest.mark.parametrize("kernel", [native.KERNEL_GENERIC, native.KERNEL_AUTO])
 @pytest.mark.parametrize("kind,d,deg,q,extra", [
     (native.BOUNDED_BARRIER, 1, 5, 12, dict(u_minus=2.9, u_plus=12.1)),
     (native.EULER2D, 2, 3, 5, {}),
     (native.SHALLOW_WATER2D, 2, 3, 5, {}),
     (native.SHALLOW_WATER2D_BARRIER, 3, 4, 5, dict(barrier=0.01)),
 ])
-def test_other_closures_match_oracle(oracle, kind, d, deg, q, extra):
+def test_other_closures_match_oracle(oracle, kind, d, deg, q, extra, kernel):
     b = native.Basis(d, [deg], q)
     S = native.STATES[kind]
     print("case: kind", kind, "M", b.M, "Q", b.Q, flush=True)
@@ -104,7 +114,10 @@ def test_other_closures_match_oracle(oracle, kind, d, deg, q, extra):
     for j in range(n):
         lam0[j, 0] = oracle.DS(c, u[j, 0])
     cl = native.GpuClosure(kind, b, epsilon=1e-9, safe_line_search=1, **extra)
+    cl.set_kernel(kernel)
     nfail, lam, st, it = _run_gpu(cl, u, lam0)
+    expect_wave = kernel == native.KERNEL_AUTO and b.M * native.STATES[kind] < 64 and b.Q <= 64
+    assert cl.last_kernel() == (native.KERNEL_WAVE if expect_wave else native.KERNEL_GENERIC)
     lam_o = lam0.copy()
     st_o, it_o = oracle.solve_closure(c, b.phi, b.phiw, u, lam_o)
     assert (st_o == 0).all()

@@ -1,0 +1,361 @@
+// closure_wave.hip -- batched FP64 Newton dual closure, one wavefront per
+// cell, specialised at compile time for the (closure, M, Q) of a config with
+// n = S*M <= 63 duals and Q <= 64 quadrature points.
+//
+// Same algorithm as closure_fp64.hip (Closure::SolveClosure / SolveClosureSafe,
+// closure.cpp:133-193, :297-339), laid out for wave64:
+//   * lane k owns quadrature point k: it keeps phi(k, :) in registers and
+//     evaluates Lambda(k) = lambda^T phi(k), U and DU there;
+//   * lane e owns dual e = l*M + i of the gradient, trial gradient and Newton
+//     direction (registers), so the line search never touches LDS for them;
+//   * Hessian (closure.cpp:255-271): lanes own basis pairs (j <= i) and sum all
+//     S(S+1)/2 state blocks over k, reading DU_k as LDS broadcasts;
+//   * Cholesky (dposv 'U', vectorspace.cpp:152-171): lane c holds column c of
+//     the upper triangle in registers, lane n holds the right-hand side, so the
+//     right-looking elimination also performs the forward substitution; row j
+//     of U is broadcast through a double-buffered LDS row per step;
+//   * back substitution: lane r holds y_r, column i of U is read from LDS.
+#include <hip/hip_runtime.h>
+
+#include "uqk_device.h"
+#include "uqk_internal.h"
+
+namespace uqk {
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    long long x = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)x, lane);
+    int hi = __builtin_amdgcn_readlane((int)(x >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int KIND, int M, int Q>
+struct WaveShape {
+    static constexpr int S = KindInfo<KIND>::S;
+    static constexpr int N = S * M;          // duals per cell
+    static constexpr int P = S * (S + 1) / 2; // distinct DU entries
+    static constexpr int QP = (Q + 1) & ~1;   // padded row length (b128 reads)
+    static constexpr int NPAIR = M * (M + 1) / 2;
+    static constexpr int HP = N * (N + 1) / 2; // packed upper triangle
+    // per-wave LDS doubles
+    static constexpr int LAM = 0;
+    static constexpr int LAMNEW = LAM + S * M;
+    static constexpr int UU = LAMNEW + S * M;
+    static constexpr int UB = UU + S * M;        // U values, S rows of QP
+    static constexpr int DUB = UB + S * QP;      // DU values, Q rows of P
+    static constexpr int HS = DUB + Q * P;       // packed H / U factor
+    static constexpr int ROW = HS + HP;          // 2 x 64 broadcast rows
+    static constexpr int YB = ROW + 128;         // 64
+    static constexpr int DIAG = YB + 64;         // 64 reciprocal pivots
+    static constexpr int WAVE_DOUBLES = DIAG + 64;
+    static constexpr int TABLE_DOUBLES = 2 * M * QP;  // phiT, phiwT (i-major)
+};
+
+template <int KIND, int M, int Q>
+__global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
+    using W = WaveShape<KIND, M, Q>;
+    constexpr int S = W::S, N = W::N, P = W::P, QP = W::QP;
+    static_assert(N + 1 <= 64 && Q <= 64, "wave kernel needs n < 64 and Q <= 64");
+    extern __shared__ double smem[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    double* phiT = smem;              // phiT[i*QP + k]  = phi(k, i)
+    double* phiwT = smem + M * QP;    // phiwT[i*QP + k] = phiw(k, i)
+    for (int t = threadIdx.x; t < M * QP; t += blockDim.x) {
+        int i = t / QP, k = t - i * QP;
+        phiT[t] = k < Q ? a.phi[k * M + i] : 0.0;
+        phiwT[t] = k < Q ? a.phiw[k * M + i] : 0.0;
+    }
+    __syncthreads();
+    double* wb = smem + W::TABLE_DOUBLES + wave * W::WAVE_DOUBLES;
+    double* lamA = wb + W::LAM;
+    double* lamB = wb + W::LAMNEW;
+    double* uu = wb + W::UU;
+    double* Ub = wb + W::UB;
+    double* DUb = wb + W::DUB;
+    double* Hs = wb + W::HS;
+    double* row = wb + W::ROW;
+    double* yb = wb + W::YB;
+    double* rdiag = wb + W::DIAG;
+
+    // lane k's row of phi, in registers
+    const int kq = lane < Q ? lane : Q - 1;
+    double phik[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) phik[i] = a.phi[kq * M + i];
+
+    // this lane's dual index e = l*M + i (valid for lane < N)
+    const int el = lane < N ? lane / M : 0;
+    const int ei = lane < N ? lane - el * M : 0;
+    // this lane's basis pairs (j <= i), pair = lane + 64 * pass
+    int pj[2], pi[2];
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+        int e = lane + 64 * pass;
+        if (e >= W::NPAIR) e = 0;
+        int i = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+        while ((i + 1) * (i + 2) / 2 <= e) ++i;
+        while (i * (i + 1) / 2 > e) --i;
+        pi[pass] = i;
+        pj[pass] = e - i * (i + 1) / 2;
+    }
+    const ClosureParams& c = a.c;
+    const int SM = S * M;
+
+    for (;;) {
+        long cell = 0;
+        if (lane == 0) cell = (long)atomicAdd(a.counter, 1ull);
+        cell = __shfl(cell, 0, 64);
+        if (cell >= a.ncell) break;
+        const double* ug = a.u + cell * SM;
+        double* lg = a.lam + cell * SM;
+        for (int t = lane; t < SM; t += 64) {
+            uu[t] = ug[t];
+            lamA[t] = lg[t];
+        }
+        wave_sync();
+        const double ue = lane < N ? uu[ei * S + el] : 0.0;
+
+        double Lk[S];  // Lambda at this lane's point for the last gradient
+        // gradient at X (LDS); returns ||g||, leaves g_e in gout, Lambda in Lk
+        auto gradient = [&](const double* X, double& gout) -> double {
+            double L[S], Uv[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) L[s] = 0.0;
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+#pragma unroll
+                for (int s = 0; s < S; ++s) L[s] += X[i * S + s] * phik[i];
+            closure_U<KIND>(c, L, Uv);
+            if (lane < Q) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) Ub[s * QP + lane] = Uv[s];
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) Lk[s] = L[s];
+            wave_sync();
+            double acc = 0.0;
+            const double* ur = Ub + el * QP;
+            const double* wr = phiwT + ei * QP;
+#pragma unroll 7
+            for (int k = 0; k < Q; ++k) acc += ur[k] * wr[k];
+            double g = lane < N ? acc - ue : 0.0;
+            gout = g;
+            double nrm = sqrt(wave_sum(g * g));
+            wave_sync();
+            return nrm;
+        };
+
+        // Hessian at the iterate of the last gradient -> Hs (packed upper)
+        auto hessian = [&]() {
+            if (lane < Q) {
+                double y[P];
+                closure_DU<KIND>(c, Lk, y);
+#pragma unroll
+                for (int p = 0; p < P; ++p) DUb[lane * P + p] = y[p];
+            }
+            wave_sync();
+#pragma unroll
+            for (int pass = 0; pass < 2; ++pass) {
+                if (64 * pass >= W::NPAIR) break;
+                const int e = lane + 64 * pass;
+                const int j = pj[pass], i = pi[pass];
+                double acc[P];
+#pragma unroll
+                for (int p = 0; p < P; ++p) acc[p] = 0.0;
+                const double* fj = phiT + j * QP;
+                const double* wi = phiwT + i * QP;
+#pragma unroll 7
+                for (int k = 0; k < Q; ++k) {
+                    double pk = fj[k] * wi[k];
+#pragma unroll
+                    for (int p = 0; p < P; ++p) acc[p] += pk * DUb[k * P + p];
+                }
+                if (e < W::NPAIR) {
+#pragma unroll
+                    for (int l = 0; l < S; ++l)
+#pragma unroll
+                        for (int m = 0; m <= l; ++m) {
+                            double v = acc[l * (l + 1) / 2 + m];
+                            int r1 = m * M + j, c1 = l * M + i;
+                            Hs[c1 * (c1 + 1) / 2 + r1] = v;
+                            if (m < l) {
+                                int r2 = m * M + i, c2 = l * M + j;
+                                Hs[c2 * (c2 + 1) / 2 + r2] = v;
+                            }
+                        }
+                }
+            }
+            wave_sync();
+        };
+
+        // d <- H^{-1} g; returns false if H is not positive definite
+        auto cholesky_solve = [&](double g, double& d) -> bool {
+            // lane c < N: column c of H; lane N: right-hand side
+            double Hc[N];
+            yb[lane] = g;
+            wave_sync();
+            if (lane < N) {
+                const int base = lane * (lane + 1) / 2;
+#pragma unroll
+                for (int r = 0; r < N; ++r) Hc[r] = (r <= lane) ? Hs[base + (r <= lane ? r : 0)] : 0.0;
+            } else {
+#pragma unroll
+                for (int r = 0; r < N; ++r) Hc[r] = yb[r];
+            }
+            bool ok = true;
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const double hjj = readlane_f64(Hc[j], j);
+                if (!(hjj > 0.0)) {
+                    ok = false;
+                    break;
+                }
+                const double ujj = sqrt(hjj);
+                const double rinv = 1.0 / ujj;
+                if (lane == 0) rdiag[j] = rinv;
+                Hc[j] = (lane == j) ? ujj : Hc[j] * rinv;
+                double* rw = row + (j & 1) * 64;
+                rw[lane] = Hc[j];
+                wave_sync();
+                const double hj = Hc[j];
+#pragma unroll
+                for (int r = j + 1; r < N; ++r) Hc[r] -= rw[r] * hj;
+            }
+            if (!ok) return false;
+            // U columns -> Hs (packed), y -> yb
+            if (lane < N) {
+                const int base = lane * (lane + 1) / 2;
+#pragma unroll
+                for (int r = 0; r < N; ++r)
+                    if (r <= lane) Hs[base + r] = Hc[r];
+            } else if (lane == N) {
+#pragma unroll
+                for (int r = 0; r < N; ++r) yb[r] = Hc[r];
+            }
+            wave_sync();
+            double yr = lane < N ? yb[lane] : 0.0;
+#pragma unroll
+            for (int i = N - 1; i >= 0; --i) {
+                const double xi = readlane_f64(yr, i) * rdiag[i];
+                if (lane < i) yr -= Hs[i * (i + 1) / 2 + lane] * xi;
+                if (lane == i) yr = xi;
+            }
+            d = yr;
+            wave_sync();
+            return true;
+        };
+
+        double* lam = lamA;
+        double* lamnew = lamB;
+        double g = 0.0, gnew = 0.0, d = 0.0;
+        int ngrad = 1;
+        double ng = gradient(lam, g);
+        int status = -1;
+        int iters = 0;
+        if (!c.safe && ng < c.eps) status = UQK_STATUS_CONVERGED;
+        for (int it = 0; status < 0 && it < c.max_iter; ++it) {
+            if (it > 0) g = gnew;
+            hessian();
+            ++iters;
+            if (!cholesky_solve(g, d)) {
+                status = UQK_STATUS_NOT_SPD;
+                break;
+            }
+            if (c.max_iter == 1) {  // one-shot IPM (closure.cpp:151-154)
+                if (lane < N) lam[ei * S + el] -= c.alpha * d;
+                wave_sync();
+                status = UQK_STATUS_CONVERGED;
+                break;
+            }
+            auto trial = [&](double step) -> double {
+                if (lane < N) lamnew[ei * S + el] = lam[ei * S + el] - step * c.alpha * d;
+                wave_sync();
+                ++ngrad;
+                return gradient(lamnew, gnew);
+            };
+            double step = 1.0;
+            double nnew = trial(step);
+            int halvings = 0;
+            bool done = false;
+            while (ng < nnew || (c.safe && !isfinite(nnew))) {
+                step *= 0.5;
+                nnew = trial(step);
+                if (nnew < c.eps) {
+                    status = UQK_STATUS_CONVERGED;
+                    done = true;
+                    break;
+                }
+                if (++halvings > c.max_halvings) {
+                    status = UQK_STATUS_LINE_SEARCH;
+                    done = true;
+                    break;
+                }
+            }
+            if (done) {
+                if (status == UQK_STATUS_CONVERGED) {
+                    double* t = lam;
+                    lam = lamnew;
+                    lamnew = t;
+                }
+                break;
+            }
+            double* t = lam;
+            lam = lamnew;
+            lamnew = t;
+            ng = nnew;
+            if (!isfinite(nnew)) {
+                status = UQK_STATUS_NONFINITE;
+                break;
+            }
+            if (nnew < c.eps) status = UQK_STATUS_CONVERGED;
+        }
+        if (status < 0) status = UQK_STATUS_MAX_ITERATIONS;
+        for (int t = lane; t < SM; t += 64) lg[t] = lam[t];
+        if (lane == 0) {
+            if (a.status) a.status[cell] = status;
+            if (a.iters) a.iters[cell] = iters;
+            if (status != UQK_STATUS_CONVERGED) atomicAdd(a.nfail, 1ull);
+            atomicAdd(a.nfail + 1, (unsigned long long)iters);
+            atomicAdd(a.nfail + 2, (unsigned long long)ngrad);
+        }
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// dispatch over the compiled shapes
+// ---------------------------------------------------------------------------
+template <int KIND, int M, int Q>
+static bool wave_try(int kind, int m, int q, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
+                     hipError_t* err) {
+    if (kind != KIND || m != M || q != Q) return false;
+    using W = WaveShape<KIND, M, Q>;
+    const size_t tb = sizeof(double) * W::TABLE_DOUBLES, wbytes = sizeof(double) * W::WAVE_DOUBLES;
+    int waves = 4;
+    while (waves > 1 && tb + waves * wbytes > (size_t)max_lds) --waves;
+    if (tb + waves * wbytes > (size_t)max_lds) return false;
+    const int threads = 64 * waves;
+    const size_t shmem = tb + waves * wbytes;
+    int occ = 0;
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, newton_wave_kernel<KIND, M, Q>, threads, shmem);
+    if (occ <= 0) occ = 1;
+    long want = (a.ncell + waves - 1) / waves;
+    int blocks = (int)(want < (long)occ * num_cu ? want : (long)occ * num_cu);
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL((newton_wave_kernel<KIND, M, Q>), dim3(blocks), dim3(threads), shmem, st, a);
+    *err = hipGetLastError();
+    return true;
+}
+
+bool newton_wave_launch(int kind, int M, int Q, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
+                        hipError_t* err) {
+    if (a.nt_cell) return false;  // per-cell orders use the generic kernel
+    return wave_try<UQK_BOUNDED_BARRIER, 6, 12>(kind, M, Q, a, max_lds, num_cu, st, err) ||
+           wave_try<UQK_EULER1D, 15, 49>(kind, M, Q, a, max_lds, num_cu, st, err) ||
+           wave_try<UQK_EULER2D, 10, 25>(kind, M, Q, a, max_lds, num_cu, st, err) ||
+           wave_try<UQK_SHALLOW_WATER2D, 10, 25>(kind, M, Q, a, max_lds, num_cu, st, err) ||
+           wave_try<UQK_SHALLOW_WATER2D_BARRIER, 10, 25>(kind, M, Q, a, max_lds, num_cu, st, err);
+}
+
+}  // namespace uqk

@@ -30,6 +30,8 @@ struct uqk_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_closure_ms = 0.0;
     unsigned long long last_stats[3] = {0, 0, 0};
+    int kernel = UQK_KERNEL_AUTO;
+    int last_kernel = UQK_KERNEL_AUTO;
     size_t max_lds = 0;
     int num_cu = 0;
     std::string err;
@@ -117,6 +119,18 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     a.iters = iters;
     a.counter = ctx->d_counters;
     a.nfail = ctx->d_counters + 1;
+    if (ctx->kernel != UQK_KERNEL_GENERIC) {
+        hipError_t e = hipSuccess;
+        UQK_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        if (uqk::newton_wave_launch(ctx->kind, ctx->M, ctx->Q, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream, &e)) {
+            UQK_HIP(ctx, e);
+            UQK_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+            ctx->last_kernel = UQK_KERNEL_WAVE;
+            return 0;
+        }
+        if (ctx->kernel == UQK_KERNEL_WAVE) return fail(ctx, -3, "no specialised closure kernel for this shape");
+    }
+    ctx->last_kernel = UQK_KERNEL_GENERIC;
     a.wave_doubles = uqk::newton_fp64_wave_doubles(ctx->S, ctx->M, ctx->Q);
     const size_t wave_bytes = sizeof(double) * (size_t)a.wave_doubles;
     const size_t table_bytes = sizeof(double) * 2 * (size_t)ctx->Q * ctx->M;
@@ -252,6 +266,15 @@ int uqk_set_stream(uqk_ctx* ctx, void* s) {
     ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
     return 0;
 }
+
+int uqk_set_closure_kernel(uqk_ctx* ctx, int32_t kernel) {
+    if (!ctx) return -1;
+    if (kernel < UQK_KERNEL_AUTO || kernel > UQK_KERNEL_WAVE) return fail(ctx, -1, "unknown closure kernel");
+    ctx->kernel = kernel;
+    return 0;
+}
+
+int32_t uqk_last_closure_kernel(const uqk_ctx* ctx) { return ctx ? ctx->last_kernel : -1; }
 
 int uqk_synchronize(uqk_ctx* ctx) {
     if (!ctx) return -1;

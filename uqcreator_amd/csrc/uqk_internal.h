@@ -27,6 +27,8 @@ struct NewtonArgs {
     int wave_doubles;            // LDS doubles per wave
 };
 
+bool newton_wave_launch(int kind, int M, int Q, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
+                        hipError_t* err);
 int newton_fp64_wave_doubles(int S, int M, int Q);
 int newton_fp64_occupancy(int kind, bool tlds, int threads, size_t shmem);
 hipError_t newton_fp64_launch(int kind, const NewtonArgs& a, bool tlds, int blocks, int threads, size_t shmem,

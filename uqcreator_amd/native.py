@@ -17,6 +17,7 @@ FP64, MIXED = 0, 1
 BURGERS, EULER2D_PROBLEM, SHALLOW_WATER2D_PROBLEM = range(3)
 FLUX_REFERENCE, FLUX_LAX_FRIEDRICHS = 0, 1
 BC_NONE, BC_DIRICHLET, BC_NOSLIP, BC_SWWALL = range(4)
+KERNEL_AUTO, KERNEL_GENERIC, KERNEL_WAVE = range(3)
 STATES = {BOUNDED_BARRIER: 1, EULER1D: 3, EULER2D: 4, SHALLOW_WATER2D: 3, SHALLOW_WATER2D_BARRIER: 3}
 # Newton damping per closure (boundedbarrier.cpp:4, eulerclosure.cpp:3,
 # eulerclosure2d.cpp:3, shallowwaterclosure2d.cpp:3-6)
@@ -56,6 +57,8 @@ SIGNATURES = {
     "uqk_last_error": (C.c_char_p, [C.c_void_p]),
     "uqk_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
     "uqk_synchronize": (C.c_int, [C.c_void_p]),
+    "uqk_set_closure_kernel": (C.c_int, [C.c_void_p, C.c_int32]),
+    "uqk_last_closure_kernel": (C.c_int32, [C.c_void_p]),
     "uqk_closure_solve": (C.c_int64, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p]),
     "uqk_closure_solve_host": (C.c_int64, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -262,6 +265,13 @@ class GpuClosure:
         if rc < 0:
             raise RuntimeError("%s: %s (%d)" % (what, lib().uqk_last_error(self.h).decode(), rc))
         return rc
+
+    def set_kernel(self, kernel):
+        """KERNEL_AUTO, KERNEL_GENERIC or KERNEL_WAVE."""
+        self._check(lib().uqk_set_closure_kernel(self.h, kernel), "uqk_set_closure_kernel")
+
+    def last_kernel(self):
+        return lib().uqk_last_closure_kernel(self.h)
 
     def set_stream(self, stream_handle):
         self._check(lib().uqk_set_stream(self.h, C.c_void_p(stream_handle)), "uqk_set_stream")
