@@ -9,11 +9,12 @@
 //   * lane e owns dual e = l*M + i of the gradient, trial gradient and Newton
 //     direction (registers), so the line search never touches LDS for them;
 //   * Hessian (closure.cpp:255-271): lanes own basis pairs (j <= i) and sum all
-//     S(S+1)/2 state blocks over k, reading DU_k as LDS broadcasts;
+//     S(S+1)/2 state blocks over k; DU_k is computed by lane k and reaches the
+//     other lanes as SGPR operands (v_readlane), which costs no LDS bandwidth;
 //   * Cholesky (dposv 'U', vectorspace.cpp:152-171): lane c holds column c of
 //     the upper triangle in registers, lane n holds the right-hand side, so the
 //     right-looking elimination also performs the forward substitution; row j
-//     of U is broadcast through a double-buffered LDS row per step;
+//     of U is broadcast through SGPRs (v_readlane) at every step;
 //   * back substitution: lane r holds y_r, column i of U is read from LDS.
 #include <hip/hip_runtime.h>
 
@@ -42,10 +43,8 @@ struct WaveShape {
     static constexpr int LAMNEW = LAM + S * M;
     static constexpr int UU = LAMNEW + S * M;
     static constexpr int UB = UU + S * M;        // U values, S rows of QP
-    static constexpr int DUB = UB + S * QP;      // DU values, Q rows of P
-    static constexpr int HS = DUB + Q * P;       // packed H / U factor
-    static constexpr int ROW = HS + HP;          // 2 x 64 broadcast rows
-    static constexpr int YB = ROW + 128;         // 64
+    static constexpr int HS = UB + S * QP;       // packed H / U factor
+    static constexpr int YB = HS + HP;           // 64
     static constexpr int DIAG = YB + 64;         // 64 reciprocal pivots
     static constexpr int WAVE_DOUBLES = DIAG + 64;
     static constexpr int TABLE_DOUBLES = 2 * M * QP;  // phiT, phiwT (i-major)
@@ -72,9 +71,7 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
     double* lamB = wb + W::LAMNEW;
     double* uu = wb + W::UU;
     double* Ub = wb + W::UB;
-    double* DUb = wb + W::DUB;
     double* Hs = wb + W::HS;
-    double* row = wb + W::ROW;
     double* yb = wb + W::YB;
     double* rdiag = wb + W::DIAG;
 
@@ -119,6 +116,7 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
         double Lk[S];  // Lambda at this lane's point for the last gradient
         // gradient at X (LDS); returns ||g||, leaves g_e in gout, Lambda in Lk
         auto gradient = [&](const double* X, double& gout) -> double {
+            // phase: gradient
             double L[S], Uv[S];
 #pragma unroll
             for (int s = 0; s < S; ++s) L[s] = 0.0;
@@ -143,40 +141,51 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
             gout = g;
             double nrm = sqrt(wave_sum(g * g));
             wave_sync();
+            // phase end: gradient
             return nrm;
         };
 
-        // Hessian at the iterate of the last gradient -> Hs (packed upper)
+        // Hessian at the iterate of the last gradient -> Hs (packed upper).
+        // DU at point k lives in lane k; it reaches every lane through
+        // v_readlane (an SGPR operand), not through LDS.
         auto hessian = [&]() {
-            if (lane < Q) {
-                double y[P];
-                closure_DU<KIND>(c, Lk, y);
+            // phase: hessian
+            double y[P];
+            closure_DU<KIND>(c, Lk, y);
+            constexpr int PASSES = W::NPAIR > 64 ? 2 : 1;
+            double acc[PASSES][P];
 #pragma unroll
-                for (int p = 0; p < P; ++p) DUb[lane * P + p] = y[p];
-            }
-            wave_sync();
+            for (int q = 0; q < PASSES; ++q)
 #pragma unroll
-            for (int pass = 0; pass < 2; ++pass) {
-                if (64 * pass >= W::NPAIR) break;
-                const int e = lane + 64 * pass;
-                const int j = pj[pass], i = pi[pass];
-                double acc[P];
-#pragma unroll
-                for (int p = 0; p < P; ++p) acc[p] = 0.0;
-                const double* fj = phiT + j * QP;
-                const double* wi = phiwT + i * QP;
+                for (int p = 0; p < P; ++p) acc[q][p] = 0.0;
+            const double* fj0 = phiT + pj[0] * QP;
+            const double* wi0 = phiwT + pi[0] * QP;
+            const double* fj1 = phiT + pj[PASSES - 1] * QP;
+            const double* wi1 = phiwT + pi[PASSES - 1] * QP;
 #pragma unroll 7
-                for (int k = 0; k < Q; ++k) {
-                    double pk = fj[k] * wi[k];
+            for (int k = 0; k < Q; ++k) {
+                double du[P];
 #pragma unroll
-                    for (int p = 0; p < P; ++p) acc[p] += pk * DUb[k * P + p];
+                for (int p = 0; p < P; ++p) du[p] = readlane_f64(y[p], k);
+                const double pk0 = fj0[k] * wi0[k];
+#pragma unroll
+                for (int p = 0; p < P; ++p) acc[0][p] += pk0 * du[p];
+                if constexpr (PASSES == 2) {
+                    const double pk1 = fj1[k] * wi1[k];
+#pragma unroll
+                    for (int p = 0; p < P; ++p) acc[PASSES - 1][p] += pk1 * du[p];
                 }
+            }
+#pragma unroll
+            for (int q = 0; q < PASSES; ++q) {
+                const int e = lane + 64 * q;
+                const int j = pj[q], i = pi[q];
                 if (e < W::NPAIR) {
 #pragma unroll
                     for (int l = 0; l < S; ++l)
 #pragma unroll
                         for (int m = 0; m <= l; ++m) {
-                            double v = acc[l * (l + 1) / 2 + m];
+                            double v = acc[q][l * (l + 1) / 2 + m];
                             int r1 = m * M + j, c1 = l * M + i;
                             Hs[c1 * (c1 + 1) / 2 + r1] = v;
                             if (m < l) {
@@ -187,10 +196,12 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
                 }
             }
             wave_sync();
+            // phase end: hessian
         };
 
         // d <- H^{-1} g; returns false if H is not positive definite
         auto cholesky_solve = [&](double g, double& d) -> bool {
+            // phase: factor
             // lane c < N: column c of H; lane N: right-hand side
             double Hc[N];
             yb[lane] = g;
@@ -215,14 +226,14 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
                 const double rinv = 1.0 / ujj;
                 if (lane == 0) rdiag[j] = rinv;
                 Hc[j] = (lane == j) ? ujj : Hc[j] * rinv;
-                double* rw = row + (j & 1) * 64;
-                rw[lane] = Hc[j];
-                wave_sync();
                 const double hj = Hc[j];
+                // row j of U, U(j, r) held by lane r, broadcast through SGPRs
 #pragma unroll
-                for (int r = j + 1; r < N; ++r) Hc[r] -= rw[r] * hj;
+                for (int r = j + 1; r < N; ++r) Hc[r] -= readlane_f64(hj, r) * hj;
             }
             if (!ok) return false;
+            // phase end: factor
+            // phase: backsub
             // U columns -> Hs (packed), y -> yb
             if (lane < N) {
                 const int base = lane * (lane + 1) / 2;
@@ -243,6 +254,7 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
             }
             d = yr;
             wave_sync();
+            // phase end: backsub
             return true;
         };
 
