@@ -1,0 +1,206 @@
+"""GPU sweeps (u at the quadrature points, reconstruction, time step, flux +
+update, mean/variance) and whole time loops against the FP64 CPU oracle."""
+import numpy as np
+import pytest
+
+from uqcreator_amd import mesh as meshes
+from uqcreator_amd import native, solver, workloads
+
+pytestmark = pytest.mark.gpu
+
+OK = {native.BOUNDED_BARRIER: "BB", native.EULER1D: "EULER1D", native.EULER2D: "EULER2D",
+      native.SHALLOW_WATER2D: "SW2D", native.SHALLOW_WATER2D_BARRIER: "SW2D_BARRIER"}
+
+
+def _oracle_closure(oracle, kind, **kw):
+    return oracle.closure(getattr(oracle, OK[kind]), **kw)
+
+
+def _oracle_mesh(oracle, m):
+    return oracle.MeshArrays(m.nbr, m.unit_normal, m.normal, m.area, m.dx_param, m.bc, m.bnormal)
+
+
+def _rand_duals(oracle, kind, b, n, rng, amp=0.02):
+    c = _oracle_closure(oracle, kind)
+    lam = np.zeros((n, b.M, c.S))
+    for j in range(n):
+        if kind == native.EULER2D:
+            rho, vx, vy, p = rng.uniform(0.5, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.5, 2)
+            cons = np.array([rho, rho * vx, rho * vy, p / 0.4 + 0.5 * rho * (vx * vx + vy * vy)])
+        elif kind == native.EULER1D:
+            rho, vx, p = rng.uniform(0.5, 2), rng.uniform(-1, 1), rng.uniform(0.5, 2)
+            cons = np.array([rho, rho * vx, p / 0.4 + 0.5 * rho * vx * vx])
+        elif kind == native.BOUNDED_BARRIER:
+            lam[j, 0, 0] = rng.uniform(-2, 2)
+            lam[j, 1:, 0] = rng.normal(0, 0.2, b.M - 1)
+            continue
+        else:
+            h, vx, vy = rng.uniform(1, 3), rng.uniform(-1, 1), rng.uniform(-1, 1)
+            cons = np.array([h, h * vx, h * vy])
+        lam[j, 0] = oracle.DS(c, cons)
+        lam[j, 1:] = rng.normal(0, amp, (b.M - 1, c.S)) * np.abs(lam[j, 0])
+    return lam
+
+
+@pytest.mark.parametrize("kind,d,deg,q", [(native.EULER2D, 2, 3, 5), (native.EULER1D, 2, 4, 7),
+                                          (native.BOUNDED_BARRIER, 1, 5, 12),
+                                          (native.SHALLOW_WATER2D_BARRIER, 3, 4, 5)])
+def test_uq_reconstruct_mean_var(oracle, kind, d, deg, q):
+    b = native.Basis(d, [deg], q)
+    rng = np.random.default_rng(3)
+    n = 300
+    lam = _rand_duals(oracle, kind, b, n, rng)
+    print("case: sweeps kind", kind, "M", b.M, "Q", b.Q, flush=True)
+    kw = dict(barrier=0.01) if kind == native.SHALLOW_WATER2D_BARRIER else {}
+    cl = native.GpuClosure(kind, b, **kw)
+    c = _oracle_closure(oracle, kind, **kw)
+    dl = native.DeviceArray.from_host(lam)
+    duq = native.DeviceArray((n, b.Q, cl.S))
+    cl.eval_uq(dl, duq)
+    uq = duq.download()
+    uq_o = oracle.eval_uq(c, b.phi, lam)
+    np.testing.assert_allclose(uq, uq_o, rtol=1e-12, atol=0)
+    du = native.DeviceArray((n, b.M, cl.S))
+    cl.reconstruct(duq, du)
+    np.testing.assert_allclose(du.download(), oracle.reconstruct(cl.S, b.phiw, uq_o), rtol=1e-11,
+                               atol=1e-13 * np.abs(uq_o).max())
+    mv = native.DeviceArray((n, 2 * cl.S))
+    cl.mean_var(dl, mv)
+    mv_o = oracle.mean_var(c, b.phi, b.phiw, lam)
+    np.testing.assert_allclose(mv.download(), mv_o, rtol=1e-11, atol=1e-14 * np.abs(mv_o).max())
+
+
+def _perturbed_state(oracle, kind, b, m, rng, base_u):
+    """moments of a smooth random state on the mesh, via the oracle closure"""
+    n = m.n_cells
+    c = _oracle_closure(oracle, kind)
+    lam = np.zeros((n, b.M, c.S))
+    for j in range(n):
+        lam[j, 0] = oracle.DS(c, base_u * (1 + 0.05 * rng.normal(size=c.S)))
+        lam[j, 1:] = rng.normal(0, 0.005, (b.M - 1, c.S)) * np.abs(lam[j, 0])
+    uq = oracle.eval_uq(c, b.phi, lam)
+    return np.ascontiguousarray(uq), lam
+
+
+@pytest.mark.parametrize("case", ["burgers_lf", "burgers_upwind", "euler_hll", "euler_rusanov", "sw_hll_friction"])
+def test_dt_and_advance(oracle, case):
+    rng = np.random.default_rng(5)
+    if case.startswith("burgers"):
+        kind, b = native.BOUNDED_BARRIER, native.Basis(1, [5], 12)
+        m = meshes.mesh1d(200, 0.0, 3.0)
+        pk = native.BURGERS
+        flux = native.FLUX_LAX_FRIEDRICHS if case.endswith("lf") else native.FLUX_REFERENCE
+        lam = _rand_duals(oracle, kind, b, m.n_cells, rng)
+        c = _oracle_closure(oracle, kind)
+        uq = np.ascontiguousarray(oracle.eval_uq(c, b.phi, lam))
+        extra = {}
+    elif case.startswith("euler"):
+        kind, b = native.EULER2D, native.Basis(2, [3], 5)
+        m = meshes.naca_ogrid(48, 12, radius=5.0, first_height=2e-2)
+        pk = native.EULER2D_PROBLEM
+        flux = native.FLUX_REFERENCE if case.endswith("hll") else native.FLUX_LAX_FRIEDRICHS
+        uq, lam = _perturbed_state(oracle, kind, b, m, rng, np.array([1.0, 0.5, 0.1, 2.5]))
+        extra = {}
+    else:
+        kind, b = native.SHALLOW_WATER2D_BARRIER, native.Basis(3, [2], 3)
+        m = meshes.quad_grid(12, 10, -1, 1, -1, 1, native.BC_SWWALL)
+        pk = native.SHALLOW_WATER2D_PROBLEM
+        flux = native.FLUX_REFERENCE
+        c = _oracle_closure(oracle, kind, barrier=0.01)
+        lam = np.zeros((m.n_cells, b.M, 3))
+        for j in range(m.n_cells):
+            lam[j, 0] = oracle.DS(c, np.array([1.5, 0.3, -0.2]) * (1 + 0.1 * rng.normal(size=3)))
+            lam[j, 1:] = rng.normal(0, 0.01, (b.M - 1, 3))
+        uq = np.ascontiguousarray(oracle.eval_uq(c, b.phi, lam))
+        extra = dict(barrier=0.01)
+    print("case:", case, "cells", m.n_cells, flush=True)
+    fr = 0.01 if case == "sw_hll_friction" else 0.0
+    prob = native.problem(pk, flux=flux, cfl=0.4, friction=fr, friction_dim=2 if fr else -1)
+    oprob = oracle.problem(pk, flux=flux, cfl=0.4, friction=fr, friction_dim=2 if fr else -1)
+    cl = native.GpuClosure(kind, b, **extra)
+    dm = native.DeviceMesh(m)
+    om = _oracle_mesh(oracle, m)
+    duq = native.DeviceArray.from_host(uq)
+    dt = cl.compute_dt(prob, dm, duq)
+    dt_o = min(oracle.compute_dt(oprob, uq[j], m.dx_param[j]) for j in range(m.n_cells))
+    assert abs(dt - dt_o) <= 1e-14 * dt_o
+    u = oracle.reconstruct(cl.S, b.phiw, uq)
+    du = native.DeviceArray.from_host(u)
+    dun = native.DeviceArray.from_host(np.zeros_like(u))
+    cl.advance(prob, dm, dt, du, duq, dun)
+    un_o = oracle.advance(oprob, om, b.phiw, b.xi, dt, u, uq)
+    scale = np.abs(u).max(axis=(0, 1))
+    assert (np.abs(dun.download() - un_o) / scale).max() < 1e-12
+
+
+def _run_both(oracle, kind, b, m, u0, lam0, prob_kw, clo_kw, t_end, max_steps):
+    prob = native.problem(**prob_kw)
+    okw = dict(prob_kw)
+    oprob = oracle.problem(okw.pop("kind"), **{("g" if k == "gravity" else k): v for k, v in okw.items()})
+    cl = native.GpuClosure(kind, b, **clo_kw)
+    s = solver.GpuMomentSolver(cl, prob, m)
+    s.set_state(u0, lam0)
+    s.run(t_end, max_steps)
+    ocl_kw = {{"epsilon": "eps", "max_iterations": "max_iter", "safe_line_search": "safe"}.get(k, k): v
+              for k, v in clo_kw.items()}
+    c = _oracle_closure(oracle, kind, **ocl_kw)
+    uo, lo = u0.copy(), lam0.copy()
+    res = oracle.run(c, oprob, _oracle_mesh(oracle, m), b.phi, b.phiw, b.xi, uo, lo, t_end, max_steps)
+    assert res["rc"] == 0
+    assert s.steps == res["steps"]
+    assert abs(s.t - res["t"]) <= 1e-12 * res["t"]
+    mv = s.mean_var()
+    mv_o = oracle.mean_var(c, b.phi, b.phiw, lo)
+    return s, mv, mv_o, res
+
+
+def _rel_l2(a, b, area):
+    """area-weighted relative L2 per field (momentsolver.cpp:687-749); a field
+    whose norm is below 1e-10 of the largest (a variance that is zero up to
+    rounding) is measured against that floor"""
+    num = ((a - b) ** 2 * area[:, None]).sum(0)
+    den = (b ** 2 * area[:, None]).sum(0)
+    return np.sqrt(num / np.maximum(den, 1e-20 * den.max()))
+
+
+def test_burgers_c1_time_loop(oracle):
+    """The 1D Burgers config end to end: 2000 cells, t_end 0.11, LF flux,
+    bounded barrier [2.9, 12.1], N = 5, 12 Gauss points, Newton tol 1e-10."""
+    b, m, u0, lam0 = workloads.burgers_c1(2000)
+    print("case: C1 Burgers 2000 cells t_end 0.11", flush=True)
+    s, mv, mv_o, res = _run_both(oracle, native.BOUNDED_BARRIER, b, m, u0, lam0,
+                                 dict(kind=native.BURGERS, flux=native.FLUX_LAX_FRIEDRICHS, cfl=0.5),
+                                 dict(u_minus=2.9, u_plus=12.1, epsilon=1e-10), 0.11, -1)
+    assert s.steps > 1500
+    err = _rel_l2(mv, mv_o, m.area)
+    print("C1 steps", s.steps, "rel L2 of E, Var vs oracle", err, flush=True)
+    assert err.max() < 1e-8
+    # E and Var against the exact solution (burgers.cpp:88-112 for the step
+    # profile: a shock of speed (uL + uR)/2 from x0 + 0.2 xi) -- discretisation error only
+    x = m.center[:, 0]
+    xq, wq = np.polynomial.legendre.leggauss(200)
+    shock = 0.5 + 0.2 * xq[None, :] + 0.5 * (12 + 3) * s.t
+    ue = np.where(x[:, None] < shock, 12.0, 3.0)
+    E = (ue * wq[None, :] * 0.5).sum(1)
+    inner = (x > 0.05) & (x < 2.95)
+    errE = np.sqrt((((mv[:, 0] - E) ** 2)[inner]).sum() / ((E ** 2)[inner]).sum())
+    assert errE < 0.05, errE
+
+
+def test_euler2d_naca_short_loop(oracle):
+    """2D Euler on a coarse NACA0012 O-grid (64 x 16), 10 steps, HLL and
+    Rusanov, against the oracle time loop."""
+    b, m, u0 = workloads.euler2d_c3(64, 16)
+    c = oracle.closure(oracle.EULER2D)
+    lam0 = np.zeros_like(u0)
+    for j in range(m.n_cells):
+        lam0[j, 0] = oracle.DS(c, u0[j, 0])
+    for flux in (native.FLUX_REFERENCE, native.FLUX_LAX_FRIEDRICHS):
+        print("case: Euler2D NACA 64x16 flux", flux, flush=True)
+        s, mv, mv_o, res = _run_both(oracle, native.EULER2D, b, m, u0, lam0,
+                                     dict(kind=native.EULER2D_PROBLEM, flux=flux, cfl=0.4),
+                                     dict(epsilon=1e-6), 1.0, 10)
+        assert s.steps == 10
+        err = _rel_l2(mv, mv_o, m.area)
+        print("rel L2 E/Var", err, flush=True)
+        assert err.max() < 1e-8

@@ -32,6 +32,7 @@ struct uqk_ctx {
     unsigned long long last_stats[3] = {0, 0, 0};
     int kernel = UQK_KERNEL_AUTO;
     int last_kernel = UQK_KERNEL_AUTO;
+    bool timed = false;  // ev0/ev1 bracket the last closure kernel
     size_t max_lds = 0;
     int num_cu = 0;
     std::string err;
@@ -103,6 +104,7 @@ int expected_states(int kind) {
 // Launch the Newton closure kernel; leaves the failed-cell count in d_counters[1].
 int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, double* lam, int32_t* status,
                    int32_t* iters) {
+    ctx->timed = false;
     UQK_HIP(ctx, uqk::launch_fill_u64(ctx->d_counters, 4, ctx->stream));
     if (n <= 0) return 0;
     uqk::NewtonArgs a{};
@@ -126,6 +128,7 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
             UQK_HIP(ctx, e);
             UQK_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
             ctx->last_kernel = UQK_KERNEL_WAVE;
+            ctx->timed = true;
             return 0;
         }
         if (ctx->kernel == UQK_KERNEL_WAVE) return fail(ctx, -3, "no specialised closure kernel for this shape");
@@ -162,6 +165,7 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     UQK_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     UQK_HIP(ctx, uqk::newton_fp64_launch(ctx->kind, a, tlds, blocks, threads, shmem, ctx->stream));
     UQK_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->timed = true;
     return 0;
 }
 
@@ -170,7 +174,9 @@ int64_t finish_closure(uqk_ctx* ctx) {
                                 ctx->stream));
     UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_closure_ms = ms;
+    ctx->last_closure_ms = 0.0;
+    if (ctx->timed) UQK_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (ctx->timed) ctx->last_closure_ms = ms;
     return (int64_t)ctx->last_stats[0];
 }
 

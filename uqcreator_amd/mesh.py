@@ -47,3 +47,103 @@ def mesh1d(n_cells, a, b):
     un[:, 0, 0] = -1.0
     un[:, 1, 0] = 1.0
     return MeshArrays(nbr, un, un.copy(), area, area ** 2 / area, bc, np.zeros((N, 1)), nodes[:-1, None])
+
+
+def _quad_mesh(X, Y, periodic_i, bc_j0, bc_j1, bc_i0=None, bc_i1=None):
+    """Quadrilateral mesh from node arrays X, Y of shape (ni+1, nj+1) (or
+    (ni, nj+1) with periodic_i).  Cell (i, j) has id j*ni + i and nodes
+    (i,j), (i+1,j), (i+1,j+1), (i,j+1); edge e joins node e and node e+1, so the
+    edges face j-, i+, j+, i-.  Outward normals are fixed by the centroid test
+    of Quadrangle::getOutwardNormal (quadrangle.cpp:72-83); the area is the
+    shoelace formula and dx_param uses the longest edge (the reference's
+    Quadrangle never updates maxEdge, quadrangle.cpp:58,69)."""
+    ni = X.shape[0] if periodic_i else X.shape[0] - 1
+    nj = X.shape[1] - 1
+    N = ni * nj
+    ii, jj = np.meshgrid(np.arange(ni), np.arange(nj), indexing="ij")
+    ii = ii.T.ravel()
+    jj = jj.T.ravel()
+    ip = (ii + 1) % X.shape[0] if periodic_i else ii + 1
+    px = np.stack([X[ii, jj], X[ip, jj], X[ip, jj + 1], X[ii, jj + 1]], 1)
+    py = np.stack([Y[ii, jj], Y[ip, jj], Y[ip, jj + 1], Y[ii, jj + 1]], 1)
+    cx, cy = px.mean(1), py.mean(1)
+    area = 0.5 * np.abs(np.sum(px * np.roll(py, -1, 1) - np.roll(px, -1, 1) * py, 1))
+    dx = np.roll(px, -1, 1) - px
+    dy = np.roll(py, -1, 1) - py
+    length = np.hypot(dx, dy)
+    nx, ny = dy.copy(), -dx.copy()
+    flip = nx * (cx[:, None] - px) + ny * (cy[:, None] - py) > 0
+    nx[flip] *= -1
+    ny[flip] *= -1
+    nrm = np.hypot(nx, ny)
+    un = np.stack([nx / nrm, ny / nrm], -1)
+    sn = un * length[..., None]
+    cid = lambda i, j: j * ni + i
+    nbr = np.empty((N, 4), dtype=np.int64)
+    nbr[:, 0] = np.where(jj > 0, cid(ii, jj - 1), N)
+    nbr[:, 2] = np.where(jj < nj - 1, cid(ii, jj + 1), N)
+    if periodic_i:
+        nbr[:, 1] = cid((ii + 1) % ni, jj)
+        nbr[:, 3] = cid((ii - 1) % ni, jj)
+    else:
+        nbr[:, 1] = np.where(ii < ni - 1, cid(ii + 1, jj), N)
+        nbr[:, 3] = np.where(ii > 0, cid(ii - 1, jj), N)
+    bc = np.zeros(N, dtype=np.int32)
+    bc[jj == 0] = bc_j0
+    bc[jj == nj - 1] = bc_j1
+    if not periodic_i:
+        bc[ii == 0] = bc_i0
+        bc[ii == ni - 1] = bc_i1
+    # boundary normal: the edge with the largest neighbour id (cell.cpp:57-68)
+    first_max = np.argmax(nbr, axis=1)
+    bnormal = un[np.arange(N), first_max]
+    return MeshArrays(nbr, un, sn, area, length.max(1) / area, bc, bnormal, np.stack([cx, cy], 1))
+
+
+def quad_grid(nx, ny, x0, x1, y0, y1, bc_boundary):
+    """Uniform Cartesian quads on [x0,x1] x [y0,y1]; every boundary cell gets bc_boundary."""
+    xs = np.linspace(x0, x1, nx + 1)
+    ys = np.linspace(y0, y1, ny + 1)
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    return _quad_mesh(X, Y, False, bc_boundary, bc_boundary, bc_boundary, bc_boundary)
+
+
+def naca0012(x):
+    """Half thickness of NACA0012 with the closed trailing edge (-0.1036 x^4)."""
+    return 0.6 * (0.2969 * np.sqrt(x) - 0.1260 * x - 0.3516 * x ** 2 + 0.2843 * x ** 3 - 0.1036 * x ** 4)
+
+
+def naca_ogrid(ni=512, nj=128, radius=20.0, first_height=2e-3):
+    """Structured O-grid around an analytic NACA0012 of unit chord: ni cells
+    around the profile (cosine clustered at both edges, starting at the
+    trailing edge and running along the lower side first), nj cells out to a
+    circle of `radius` chords centred at mid chord, geometric stretching from
+    `first_height`.  The profile ring is a slip wall (the reference's 'noslip'
+    ghost, expliciteuler.cpp:25-43); the outer ring is Dirichlet far field."""
+    half = ni // 2
+    th = np.linspace(0.0, np.pi, half + 1)
+    # half cosine, half uniform spacing: refined at both edges without
+    # sliver cells at the trailing edge
+    xs = 0.5 * (0.5 * (1 + np.cos(th)) + (1 - th / np.pi))  # 1 -> 0
+    lower_x, lower_y = xs, -naca0012(xs)    # TE -> LE along the lower side
+    xu = xs[::-1][1:-1]                      # LE -> TE (exclusive)
+    upper_x, upper_y = xu, naca0012(xu)
+    sx = np.concatenate([lower_x, upper_x])
+    sy = np.concatenate([lower_y, upper_y])
+    assert len(sx) == ni
+    ang = np.arctan2(sy, sx - 0.5)
+    fx = 0.5 + radius * np.cos(ang)
+    fy = radius * np.sin(ang)
+    # geometric stretching s_0 = 0, s_nj = 1
+    dist = np.hypot(fx - sx, fy - sy).mean()
+    h0 = first_height / dist
+    lo, hi = 1.0 + 1e-9, 2.0
+    for _ in range(200):
+        r = 0.5 * (lo + hi)
+        tot = h0 * (r ** nj - 1) / (r - 1)
+        lo, hi = (r, hi) if tot < 1.0 else (lo, r)
+    s = np.concatenate([[0.0], h0 * (r ** np.arange(1, nj + 1) - 1) / (r - 1)])
+    s /= s[-1]
+    X = sx[:, None] + (fx - sx)[:, None] * s[None, :]
+    Y = sy[:, None] + (fy - sy)[:, None] * s[None, :]
+    return _quad_mesh(X, Y, True, native.BC_NOSLIP, native.BC_DIRICHLET)

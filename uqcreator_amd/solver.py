@@ -1,0 +1,62 @@
+"""GPU time loop of the moment solver through the C ABI.
+
+MomentSolver::Solve (momentsolver.cpp:79-162) per step: closure (warm start
+from the previous step's duals) -> uQ = U(lambda phi) -> u = uQ phiw -> dt = min
+ComputeDt -> t += dt -> ExplicitEuler::Advance -> residual of the state-0 mean.
+All of a step runs in uqk_step on the device; the host only reads back dt,
+the residual and the failed-cell count.
+"""
+import numpy as np
+
+from . import native
+
+
+class GpuMomentSolver:
+    def __init__(self, closure, problem, mesh, nt_cell=None):
+        self.cl = closure
+        self.prob = problem
+        self.mesh = mesh
+        self.dmesh = native.DeviceMesh(mesh)
+        n = mesh.n_cells
+        S, M, Q = closure.S, closure.M, closure.Q
+        self.u = native.DeviceArray((n, M, S))
+        self.lam = native.DeviceArray((n, M, S))
+        self.uq = native.DeviceArray((n, Q, S))
+        self.u_scratch = native.DeviceArray((n, M, S))
+        self.status = native.DeviceArray(n, np.int32)
+        self.iters = native.DeviceArray(n, np.int32)
+        self.nt = None if nt_cell is None else native.DeviceArray.from_host(np.asarray(nt_cell, np.int32))
+        self.t = 0.0
+        self.steps = 0
+        self.residual = None
+        self.newton_steps = 0
+
+    def set_state(self, u, lam):
+        self.u.upload(u)
+        self.lam.upload(lam)
+
+    def step(self):
+        nfail, dt, res = self.cl.step(self.prob, self.dmesh, self.u, self.lam, self.uq, self.u_scratch, self.status,
+                                      self.iters, nt_cell=self.nt)
+        if nfail:
+            st = self.status.download()
+            raise RuntimeError("closure failed in %d cells at step %d (status counts %s)"
+                               % (nfail, self.steps, np.bincount(st, minlength=5).tolist()))
+        self.t += dt
+        self.steps += 1
+        self.residual = res
+        self.newton_steps += self.cl.last_closure_stats()[0]
+        return dt, res
+
+    def run(self, t_end, max_steps=-1, min_residual=-1.0):
+        """momentsolver.cpp:79: while t < tEnd and residual > minResidual."""
+        while self.t < t_end and (max_steps < 0 or self.steps < max_steps):
+            self.step()
+            if self.residual <= min_residual:
+                break
+        return self
+
+    def mean_var(self):
+        mv = native.DeviceArray((self.mesh.n_cells, 2 * self.cl.S))
+        self.cl.mean_var(self.lam, mv, nt_cell=self.nt)
+        return mv.download()

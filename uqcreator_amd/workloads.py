@@ -62,3 +62,56 @@ def burgers_c1(n_cells=2000):
     u = project_ic(burgers_step_ic(), m.center, b)
     lam = np.zeros_like(u)  # BoundedBarrier has no DS: lambda starts at 0 (closure.cpp:289)
     return b, m, u, lam
+
+
+def euler2d_freestream_ic(gamma=1.4, R=287.87, T=273.15, p=101325.0, mach=0.8, dmach=0.02, aoa=1.25, daoa=0.5):
+    """Uncertain free stream around the profile (the reference's NACA branch,
+    euler2d.cpp:182-206): Mach = mach + dmach xi_2, angle of attack
+    aoa + daoa xi_1 degrees, SI units, the same state in every cell."""
+
+    def ic(x, xi):
+        a = np.sqrt(gamma * R * T)
+        ma = mach + (dmach * xi[:, 1] if xi.shape[1] > 1 else 0.0)
+        ang = (aoa + daoa * xi[:, 0]) * (2.0 * np.pi) / 360.0
+        rho = p / (R * T)
+        uf, vf = ma * a * np.cos(ang), ma * a * np.sin(ang)
+        st = np.stack([np.full_like(uf, rho), rho * uf, rho * vf,
+                       0.5 * rho * (uf ** 2 + vf ** 2) + p / (gamma - 1)], -1)
+        return np.broadcast_to(st[None], (x.shape[0],) + st.shape).copy()
+
+    return ic
+
+
+def euler2d_c3(ni=512, nj=128):
+    """Config: 2D Euler past NACA0012, O-grid ni x nj, total degree 3 in two
+    uncertain dims (M = 10), 5x5 Gauss (Q = 25)."""
+    from . import mesh as _mesh
+
+    b = native.Basis(2, [3], 5)
+    m = _mesh.naca_ogrid(ni, nj)
+    u = project_ic(euler2d_freestream_ic(), m.center, b)
+    return b, m, u
+
+
+def sw2d_dambreak_ic(h_right=1.0, ratio=2.0, dratio=0.5, dpos=0.1):
+    """Dam break: depth (ratio + dratio xi_1) h_right left of x = dpos xi_2,
+    h_right to the right, water at rest."""
+
+    def ic(x, xi):
+        hl = (ratio + dratio * xi[:, 0]) * h_right
+        dam = dpos * xi[:, 1]
+        h = np.where(x[:, 0:1] < dam[None, :], hl[None, :], h_right)
+        return np.stack([h, np.zeros_like(h), np.zeros_like(h)], -1)
+
+    return ic
+
+
+def sw2d_c5(n=1024, degree=4, q1=5):
+    """Config: 2D shallow-water dam break on [-1,1]^2 (n x n cells, Dirichlet
+    frame), three uncertain dims, total degree 4 (M = 35), 5^3 Gauss points."""
+    from . import mesh as _mesh
+
+    b = native.Basis(3, [degree], q1)
+    m = _mesh.quad_grid(n, n, -1.0, 1.0, -1.0, 1.0, native.BC_DIRICHLET)
+    u = project_ic(sw2d_dambreak_ic(), m.center, b)
+    return b, m, u
