@@ -165,3 +165,54 @@ def test_empty_batch():
     u = np.zeros((0, b.M, 3))
     st, it = cl.solve_host(u, u.copy())
     assert st.size == 0
+
+
+@pytest.mark.parametrize("kind,d,deg,q", [(native.EULER1D, 2, 4, 7), (native.EULER2D, 2, 3, 5),
+                                          (native.BOUNDED_BARRIER, 1, 5, 12), (native.SHALLOW_WATER2D, 2, 3, 5)])
+def test_mixed_precision_closure(oracle, kind, d, deg, q):
+    """Hessian on MFMA in FP32; the gradient, accept test and Cholesky stay
+    FP64, so the fixed point is the FP64 one to the Newton tolerance."""
+    b = native.Basis(d, [deg], q)
+    print("case: mixed kind", kind, "M", b.M, "Q", b.Q, flush=True)
+    if kind == native.EULER1D:
+        u, lam_star, lam0 = workloads.closure_euler1d(2048, basis=b, seed=21)
+        okw, gkw = dict(eps=1e-10, du_exact=1, safe=1), dict(epsilon=1e-10, du_exact=1, safe_line_search=1)
+    else:
+        rng = np.random.default_rng(kind + 40)
+        okind = {0: oracle.BB, 2: oracle.EULER2D, 3: oracle.SW2D}[kind]
+        c0 = oracle.closure(okind)
+        n = 512
+        if kind == native.BOUNDED_BARRIER:
+            lam_star = rng.normal(0, 0.3, (n, b.M, 1))
+            lam_star[:, 0] = rng.uniform(-2, 2, (n, 1))
+        else:
+            S = native.STATES[kind]
+            lam_star = np.zeros((n, b.M, S))
+            for j in range(n):
+                if kind == native.EULER2D:
+                    rho, vx, vy, p = rng.uniform(0.5, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.5, 2)
+                    cons = np.array([rho, rho * vx, rho * vy, p / 0.4 + 0.5 * rho * (vx * vx + vy * vy)])
+                else:
+                    h, vx, vy = rng.uniform(1, 3), rng.uniform(-1, 1), rng.uniform(-1, 1)
+                    cons = np.array([h, h * vx, h * vy])
+                lam_star[j, 0] = oracle.DS(c0, cons)
+                lam_star[j, 1:] = rng.normal(0, 0.02, (b.M - 1, S)) * np.abs(lam_star[j, 0])
+        uq = oracle.eval_uq(c0, b.phi, lam_star)
+        u = oracle.reconstruct(c0.S, b.phiw, np.ascontiguousarray(uq))
+        lam0 = np.zeros_like(lam_star)
+        for j in range(n):
+            lam0[j, 0] = oracle.DS(c0, u[j, 0])
+        okw, gkw = dict(eps=1e-9, safe=1), dict(epsilon=1e-9, safe_line_search=1)
+        kind_o = okind
+    c = oracle.closure(oracle.EULER1D if kind == native.EULER1D else kind_o, **okw)
+    lam_o = lam0.copy()
+    st_o, it_o = oracle.solve_closure(c, b.phi, b.phiw, u, lam_o)
+    cl = native.GpuClosure(kind, b, precision=native.MIXED, **gkw)
+    nfail, lam, st, it = _run_gpu(cl, u, lam0)
+    assert cl.last_kernel() == native.KERNEL_WAVE
+    assert nfail == 0 and (st == 0).all() and (st_o == 0).all()
+    scale = np.abs(lam_star).max(axis=1, keepdims=True)
+    assert (np.abs(lam - lam_o) / scale).max() < 1e-8
+    assert (np.abs(lam - lam_star) / scale).max() < 1e-7
+    # the FP32 Hessian changes the Newton path only slightly
+    assert it.mean() <= it_o.mean() + 1.0

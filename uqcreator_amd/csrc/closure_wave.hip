@@ -50,7 +50,12 @@ struct WaveShape {
     static constexpr int TABLE_DOUBLES = 2 * M * QP;  // phiT, phiwT (i-major)
 };
 
-template <int KIND, int M, int Q>
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// PREC 0: FP64 Hessian on the vector units.  PREC 1: Hessian on MFMA
+// (v_mfma_f32_16x16x4f32, FP32 operands and accumulation); gradient, residual,
+// accept test, Cholesky and solve stay FP64.
+template <int KIND, int M, int Q, int PREC>
 __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
     using W = WaveShape<KIND, M, Q>;
     constexpr int S = W::S, N = W::N, P = W::P, QP = W::QP;
@@ -75,11 +80,8 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
     double* yb = wb + W::YB;
     double* rdiag = wb + W::DIAG;
 
-    // lane k's row of phi, in registers
+    // lane k evaluates quadrature point k (lanes past Q repeat the last point)
     const int kq = lane < Q ? lane : Q - 1;
-    double phik[M];
-#pragma unroll
-    for (int i = 0; i < M; ++i) phik[i] = a.phi[kq * M + i];
 
     // this lane's dual index e = l*M + i (valid for lane < N)
     const int el = lane < N ? lane / M : 0;
@@ -98,6 +100,24 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
     }
     const ClosureParams& c = a.c;
     const int SM = S * M;
+
+    // MFMA operands of the Hessian blocks H_lm = Phi^T diag(DU_lm) PhiW:
+    // A(j, k) = phi(k, j) and the constant factor PhiW(k, i) of B(k, i), in the
+    // 16x16x4 lane layout (lane l: row/column l % 16, k = 4 ks + l / 16).
+    constexpr int KS = (Q + 3) / 4;
+    constexpr int TM = (M + 15) / 16;
+    // mop[(t*KS + ks)*64 + lane] = phi, mop[TM*KS*64 + ...] = phiw, FP32
+    float* mop = reinterpret_cast<float*>(smem + W::TABLE_DOUBLES + (blockDim.x >> 6) * W::WAVE_DOUBLES);
+    if constexpr (PREC == 1) {
+        for (int t2 = threadIdx.x; t2 < TM * KS * 64; t2 += blockDim.x) {
+            const int ln = t2 & 63, ks = (t2 >> 6) % KS, t = (t2 >> 6) / KS;
+            const int k = 4 * ks + (ln >> 4), j = 16 * t + (ln & 15);
+            const bool in = k < Q && j < M;
+            mop[t2] = in ? (float)a.phi[(in ? k : 0) * M + (in ? j : 0)] : 0.f;
+            mop[TM * KS * 64 + t2] = in ? (float)a.phiw[(in ? k : 0) * M + (in ? j : 0)] : 0.f;
+        }
+        __syncthreads();
+    }
 
     for (;;) {
         long cell = 0;
@@ -121,9 +141,11 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
 #pragma unroll
             for (int s = 0; s < S; ++s) L[s] = 0.0;
 #pragma unroll
-            for (int i = 0; i < M; ++i)
+            for (int i = 0; i < M; ++i) {
+                const double f = phiT[i * QP + kq];
 #pragma unroll
-                for (int s = 0; s < S; ++s) L[s] += X[i * S + s] * phik[i];
+                for (int s = 0; s < S; ++s) L[s] += X[i * S + s] * f;
+            }
             closure_U<KIND>(c, L, Uv);
             if (lane < Q) {
 #pragma unroll
@@ -148,7 +170,68 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
         // Hessian at the iterate of the last gradient -> Hs (packed upper).
         // DU at point k lives in lane k; it reaches every lane through
         // v_readlane (an SGPR operand), not through LDS.
+        auto hessian_mfma = [&]() {
+            // phase: hessian
+            double y[P];
+            closure_DU<KIND>(c, Lk, y);
+            float yf[P];
+#pragma unroll
+            for (int p = 0; p < P; ++p) yf[p] = (float)y[p];
+            f32x4 acc[P][TM][TM];
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+#pragma unroll
+                for (int tj = 0; tj < TM; ++tj)
+#pragma unroll
+                    for (int ti = 0; ti < TM; ++ti) acc[p][tj][ti] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const int kk = 4 * ks + (lane >> 4);  // < 64: DU of point kk lives in lane kk
+                float yk[P], aop[TM], bw[TM];
+#pragma unroll
+                for (int p = 0; p < P; ++p) yk[p] = __shfl(yf[p], kk, 64);
+#pragma unroll
+                for (int t = 0; t < TM; ++t) {
+                    aop[t] = mop[(t * KS + ks) * 64 + lane];
+                    bw[t] = mop[TM * KS * 64 + (t * KS + ks) * 64 + lane];
+                }
+#pragma unroll
+                for (int p = 0; p < P; ++p)
+#pragma unroll
+                    for (int ti = 0; ti < TM; ++ti) {
+                        const float bop = yk[p] * bw[ti];
+#pragma unroll
+                        for (int tj = 0; tj < TM; ++tj)
+                            acc[p][tj][ti] =
+                                __builtin_amdgcn_mfma_f32_16x16x4f32(aop[tj], bop, acc[p][tj][ti], 0, 0, 0);
+                    }
+            }
+            // D(j, i) at lane l: j = 16 tj + 4 (l / 16) + e, i = 16 ti + l % 16
+#pragma unroll
+            for (int l = 0; l < S; ++l)
+#pragma unroll
+                for (int m = 0; m <= l; ++m)
+#pragma unroll
+                    for (int tj = 0; tj < TM; ++tj)
+#pragma unroll
+                        for (int ti = 0; ti < TM; ++ti)
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) {
+                                const int j = 16 * tj + 4 * (lane >> 4) + e, i = 16 * ti + (lane & 15);
+                                if (j < M && i < M && (m < l || j <= i)) {
+                                    const int r = m * M + j, cc = l * M + i;
+                                    Hs[cc * (cc + 1) / 2 + r] = (double)acc[l * (l + 1) / 2 + m][tj][ti][e];
+                                }
+                            }
+            wave_sync();
+            // phase end: hessian
+        };
+
         auto hessian = [&]() {
+            if constexpr (PREC == 1) {
+                hessian_mfma();
+                return;
+            }
             // phase: hessian
             double y[P];
             closure_DU<KIND>(c, Lk, y);
@@ -338,25 +421,37 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
 // ---------------------------------------------------------------------------
 // dispatch over the compiled shapes
 // ---------------------------------------------------------------------------
+template <int KIND, int M, int Q, int PREC>
+static void wave_go(const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st, hipError_t* err) {
+    using W = WaveShape<KIND, M, Q>;
+    const size_t mop = PREC == 1 ? sizeof(float) * 2 * ((M + 15) / 16) * ((Q + 3) / 4) * 64 : 0;
+    const size_t tb = sizeof(double) * W::TABLE_DOUBLES + mop, wbytes = sizeof(double) * W::WAVE_DOUBLES;
+    int waves = 4;
+    while (waves > 1 && tb + waves * wbytes > (size_t)max_lds) --waves;
+    const int threads = 64 * waves;
+    const size_t shmem = tb + waves * wbytes;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, newton_wave_kernel<KIND, M, Q, PREC>, threads, shmem) !=
+            hipSuccess ||
+        occ <= 0)
+        occ = 1;
+    long want = (a.ncell + waves - 1) / waves;
+    int blocks = (int)(want < (long)occ * num_cu ? want : (long)occ * num_cu);
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL((newton_wave_kernel<KIND, M, Q, PREC>), dim3(blocks), dim3(threads), shmem, st, a);
+    *err = hipGetLastError();
+}
+
 template <int KIND, int M, int Q>
 static bool wave_try(int kind, int m, int q, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
                      hipError_t* err) {
     if (kind != KIND || m != M || q != Q) return false;
     using W = WaveShape<KIND, M, Q>;
-    const size_t tb = sizeof(double) * W::TABLE_DOUBLES, wbytes = sizeof(double) * W::WAVE_DOUBLES;
-    int waves = 4;
-    while (waves > 1 && tb + waves * wbytes > (size_t)max_lds) --waves;
-    if (tb + waves * wbytes > (size_t)max_lds) return false;
-    const int threads = 64 * waves;
-    const size_t shmem = tb + waves * wbytes;
-    int occ = 0;
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, newton_wave_kernel<KIND, M, Q>, threads, shmem);
-    if (occ <= 0) occ = 1;
-    long want = (a.ncell + waves - 1) / waves;
-    int blocks = (int)(want < (long)occ * num_cu ? want : (long)occ * num_cu);
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL((newton_wave_kernel<KIND, M, Q>), dim3(blocks), dim3(threads), shmem, st, a);
-    *err = hipGetLastError();
+    if (sizeof(double) * (W::TABLE_DOUBLES + W::WAVE_DOUBLES) > (size_t)max_lds) return false;
+    if (a.prec == UQK_FP64)
+        wave_go<KIND, M, Q, 0>(a, max_lds, num_cu, st, err);
+    else
+        wave_go<KIND, M, Q, 1>(a, max_lds, num_cu, st, err);
     return true;
 }
 

@@ -121,6 +121,7 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     a.iters = iters;
     a.counter = ctx->d_counters;
     a.nfail = ctx->d_counters + 1;
+    a.prec = ctx->precision;
     if (ctx->kernel != UQK_KERNEL_GENERIC) {
         hipError_t e = hipSuccess;
         UQK_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));

@@ -25,6 +25,7 @@ struct NewtonArgs {
     unsigned long long* counter; // work-queue head, zero at launch
     unsigned long long* nfail;   // [0] failed cells, [1] Newton steps, [2] gradient evaluations; zero at launch
     int wave_doubles;            // LDS doubles per wave
+    int prec;                    // UQK_FP64 or UQK_MIXED (Hessian arithmetic)
 };
 
 bool newton_wave_launch(int kind, int M, int Q, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
