@@ -155,6 +155,17 @@ int uqk_advance(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, con
 int64_t uqk_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const int32_t* ntotal_cell,
                  double* u, double* lambda, double* uq, double* u_scratch, int32_t* status, int32_t* iters,
                  double* dt_out, double* residual_out);
+/* The time loop of MomentSolver::Solve (momentsolver.cpp:79-162) on the
+ * device: while t < t_end and fewer than max_steps steps (max_steps < 0: no
+ * limit) and no closure failure, run the step body.  With use_graph the step is
+ * captured once into a hipGraph (re-captured when any argument changes) and
+ * replayed; t, the step count and failures live on the device and are read
+ * back every 64 steps, so there is no host synchronisation inside a step.
+ * Returns the number of failed cells of the last step (>= 0) or an error. */
+int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const int32_t* ntotal_cell,
+                      double* u, double* lambda, double* uq, double* u_scratch, int32_t* status, int32_t* iters,
+                      double t0, double t_end, int64_t max_steps, int32_t use_graph, double* t_out, int64_t* steps_out,
+                      double* residual_out, int64_t* newton_steps_out);
 /* WriteMeanAndVar (momentsolver.cpp:302-328): mv is (2S) x n_cells,
  * (f, j) at [j*2S + f].  Device pointers. */
 int uqk_mean_var(uqk_ctx* ctx, int64_t n_cells, const int32_t* ntotal_cell, const double* lambda, double* mv);

@@ -28,6 +28,7 @@ __global__ void __launch_bounds__(256) newton_fp64_kernel(NewtonArgs a) {
     constexpr int S = KindInfo<KIND>::S;
     constexpr int P = S * (S + 1) / 2;
     extern __shared__ double smem[];
+    if (a.active && *a.active == 0) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int M = a.M, Q = a.Q;

@@ -90,6 +90,7 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
     constexpr int S = W::S, N = W::N, P = W::P, QP = W::QP;
     static_assert(N + 1 <= 64 && Q <= 64, "wave kernel needs n < 64 and Q <= 64");
     extern __shared__ double smem[];
+    if (a.active && *a.active == 0) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     double* phiT = smem;              // phiT[i*QP + k]  = phi(k, i)

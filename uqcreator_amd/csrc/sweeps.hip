@@ -18,9 +18,13 @@ constexpr int kPartialBlocks = 1024;
 int sweep_partial_blocks(long) { return kPartialBlocks; }
 
 // One thread per (cell, quadrature point): uq(s,k) = U_mat(sum_i lambda(s,i) phi(k,i)).
+#define UQK_SKIP_IF_INACTIVE(a) \
+    if ((a).active && *(a).active == 0) return;
+
 template <int KIND>
 __global__ void __launch_bounds__(kSweepThreads) eval_uq_kernel(SweepArgs a) {
     constexpr int S = KindInfo<KIND>::S;
+    UQK_SKIP_IF_INACTIVE(a)
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.ncell * a.Q) return;
     long cell = t / a.Q;
@@ -43,6 +47,7 @@ __global__ void __launch_bounds__(kSweepThreads) eval_uq_kernel(SweepArgs a) {
 
 // One thread per output moment (cell, i, s): u(s,i) = sum_k uq(s,k) phiw(k,i) for i < nt.
 __global__ void __launch_bounds__(kSweepThreads) reconstruct_kernel(SweepArgs a) {
+    UQK_SKIP_IF_INACTIVE(a)
     const int S = a.S, M = a.M, Q = a.Q;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.ncell * S * M) return;
@@ -84,6 +89,7 @@ __device__ __forceinline__ double block_sum(double v) {
 template <int KIND>
 __global__ void __launch_bounds__(kSweepThreads) dt_partial_kernel(SweepArgs a) {
     constexpr int S = KindInfo<KIND>::S;
+    UQK_SKIP_IF_INACTIVE(a)
     double m = 1e10;
     for (long cell = (long)blockIdx.x * blockDim.x + threadIdx.x; cell < a.ncell; cell += (long)gridDim.x * blockDim.x) {
         const double* q = a.uq + cell * S * a.Q;
@@ -95,6 +101,7 @@ __global__ void __launch_bounds__(kSweepThreads) dt_partial_kernel(SweepArgs a) 
 }
 
 __global__ void __launch_bounds__(kSweepThreads) min_finish_kernel(SweepArgs a) {
+    UQK_SKIP_IF_INACTIVE(a)
     double m = 1e10;
     for (int t = threadIdx.x; t < a.npartial; t += blockDim.x) m = fmin(m, a.partial[t]);
     m = block_min(m);
@@ -102,6 +109,7 @@ __global__ void __launch_bounds__(kSweepThreads) min_finish_kernel(SweepArgs a) 
 }
 
 __global__ void __launch_bounds__(kSweepThreads) sum_finish_kernel(SweepArgs a) {
+    UQK_SKIP_IF_INACTIVE(a)
     double m = 0.0;
     for (int t = threadIdx.x; t < a.npartial; t += blockDim.x) m += a.partial[t];
     m = block_sum(m);
@@ -115,6 +123,7 @@ __global__ void __launch_bounds__(kSweepThreads) sum_finish_kernel(SweepArgs a) 
 template <int KIND>
 __global__ void __launch_bounds__(kSweepThreads) advance_kernel(SweepArgs a) {
     constexpr int S = KindInfo<KIND>::S;
+    UQK_SKIP_IF_INACTIVE(a)
     extern __shared__ double smem[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -251,6 +260,34 @@ __global__ void __launch_bounds__(kSweepThreads) mean_var_kernel(SweepArgs a) {
         out[s] = E[s];
         out[S + s] = V[s];
     }
+}
+
+// t < t_end and steps < max_steps and no failed cell: the step runs
+__global__ void step_begin_kernel(RunState* rs, double t_end, long long max_steps) {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        rs->active = (rs->t < t_end && (max_steps < 0 || rs->steps < max_steps) && rs->nfail == 0) ? 1 : 0;
+}
+
+// t += dt (momentsolver.cpp:134), counters of the step into the run totals
+__global__ void step_end_kernel(RunState* rs, const double* scalars, const unsigned long long* counters) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && rs->active) {
+        rs->t += scalars[0];
+        rs->residual = scalars[1];
+        rs->steps += 1;
+        rs->nfail += (long long)counters[1];
+        rs->newton += (long long)counters[2];
+        rs->gradients += (long long)counters[3];
+    }
+}
+
+hipError_t launch_step_begin(RunState* rs, double t_end, long long max_steps, hipStream_t st) {
+    hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(64), 0, st, rs, t_end, max_steps);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_end(RunState* rs, const double* scalars, const unsigned long long* counters, hipStream_t st) {
+    hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(64), 0, st, rs, scalars, counters);
+    return hipGetLastError();
 }
 
 __global__ void fill_u64_kernel(unsigned long long* p, int n) {

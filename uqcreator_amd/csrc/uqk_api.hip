@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -33,6 +34,10 @@ struct uqk_ctx {
     int kernel = UQK_KERNEL_AUTO;
     int last_kernel = UQK_KERNEL_AUTO;
     bool timed = false;  // ev0/ev1 bracket the last closure kernel
+    bool capturing = false;  // enqueueing into a graph: no events, no syncs
+    uqk::RunState* d_run = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::string graph_key;
     size_t max_lds = 0;
     int num_cu = 0;
     std::string err;
@@ -103,7 +108,7 @@ int expected_states(int kind) {
 
 // Launch the Newton closure kernel; leaves the failed-cell count in d_counters[1].
 int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, double* lam, int32_t* status,
-                   int32_t* iters) {
+                   int32_t* iters, const int* active = nullptr) {
     ctx->timed = false;
     UQK_HIP(ctx, uqk::launch_fill_u64(ctx->d_counters, 4, ctx->stream));
     if (n <= 0) return 0;
@@ -122,14 +127,16 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     a.counter = ctx->d_counters;
     a.nfail = ctx->d_counters + 1;
     a.prec = ctx->precision;
+    a.active = active;
+    const bool ev = !ctx->capturing;
     if (ctx->kernel != UQK_KERNEL_GENERIC) {
         hipError_t e = hipSuccess;
-        UQK_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         if (uqk::newton_wave_launch(ctx->kind, ctx->M, ctx->Q, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream, &e)) {
             UQK_HIP(ctx, e);
-            UQK_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+            if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
             ctx->last_kernel = UQK_KERNEL_WAVE;
-            ctx->timed = true;
+            ctx->timed = ev;
             return 0;
         }
         if (ctx->kernel == UQK_KERNEL_WAVE) return fail(ctx, -3, "no specialised closure kernel for this shape");
@@ -163,10 +170,40 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     if (occ <= 0) occ = 1;
     long want = (n + waves - 1) / waves;
     int blocks = (int)std::min<long>(want, (long)occ * ctx->num_cu);
-    UQK_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     UQK_HIP(ctx, uqk::newton_fp64_launch(ctx->kind, a, tlds, blocks, threads, shmem, ctx->stream));
-    UQK_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    ctx->timed = true;
+    if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->timed = ev;
+    return 0;
+}
+
+// Enqueue one step body (momentsolver.cpp:83-152) on ctx->stream without any
+// host synchronisation: closure, uQ, reconstruction, dt, advance, residual.
+int enqueue_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const int32_t* nt, double* u,
+                 double* lam, double* uq, double* u_scratch, int32_t* status, int32_t* iters, const int* active) {
+    const int64_t n = mesh->n_cells;
+    int rc = launch_closure(ctx, n, nt, u, lam, status, iters, active);
+    if (rc) return rc;
+    uqk::SweepArgs a = sweep_args(ctx);
+    a.p = problem_params(prob);
+    set_mesh(a, mesh);
+    a.ncell = n;
+    a.nt_cell = nt;
+    a.lam = lam;
+    a.uq_out = uq;
+    a.active = active;
+    UQK_HIP(ctx, uqk::launch_eval_uq(ctx->kind, a, ctx->stream));
+    a.uq = uq;
+    a.u_out = u_scratch;
+    UQK_HIP(ctx, uqk::launch_reconstruct(a, ctx->stream));
+    a.scalar_out = ctx->d_scalars;
+    UQK_HIP(ctx, uqk::launch_dt(ctx->kind, a, ctx->stream));
+    a.u = u_scratch;
+    a.u_out = u;
+    a.dt_dev = ctx->d_scalars;
+    UQK_HIP(ctx, uqk::launch_advance(ctx->kind, a, ctx->stream));
+    a.scalar_out = ctx->d_scalars + 1;
+    UQK_HIP(ctx, uqk::launch_residual(a, ctx->stream));
     return 0;
 }
 
@@ -259,6 +296,8 @@ int uqk_destroy(uqk_ctx* ctx) {
     hipFree(ctx->d_counters);
     hipFree(ctx->d_partial);
     hipFree(ctx->d_scalars);
+    if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
+    hipFree(ctx->d_run);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
@@ -410,34 +449,91 @@ int64_t uqk_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, co
                  double* residual_out) {
     if (!ctx || !prob || !mesh) return -1;
     UQK_HIP(ctx, hipSetDevice(ctx->device));
-    const int64_t n = mesh->n_cells;
-    int rc = launch_closure(ctx, n, nt, u, lam, status, iters);
+    int rc = enqueue_step(ctx, prob, mesh, nt, u, lam, uq, u_scratch, status, iters, nullptr);
     if (rc) return rc;
-    uqk::SweepArgs a = sweep_args(ctx);
-    a.p = problem_params(prob);
-    set_mesh(a, mesh);
-    a.ncell = n;
-    a.nt_cell = nt;
-    a.lam = lam;
-    a.uq_out = uq;
-    UQK_HIP(ctx, uqk::launch_eval_uq(ctx->kind, a, ctx->stream));
-    a.uq = uq;
-    a.u_out = u_scratch;
-    UQK_HIP(ctx, uqk::launch_reconstruct(a, ctx->stream));
-    a.scalar_out = ctx->d_scalars;
-    UQK_HIP(ctx, uqk::launch_dt(ctx->kind, a, ctx->stream));
-    a.u = u_scratch;
-    a.u_out = u;
-    a.dt_dev = ctx->d_scalars;
-    UQK_HIP(ctx, uqk::launch_advance(ctx->kind, a, ctx->stream));
-    a.scalar_out = ctx->d_scalars + 1;
-    UQK_HIP(ctx, uqk::launch_residual(a, ctx->stream));
     double sc[2];
     UQK_HIP(ctx, hipMemcpyAsync(sc, ctx->d_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     int64_t nfail = finish_closure(ctx);
     if (dt_out) *dt_out = sc[0];
     if (residual_out) *residual_out = sc[1];
     return nfail;
+}
+
+int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const int32_t* nt, double* u,
+                      double* lam, double* uq, double* u_scratch, int32_t* status, int32_t* iters, double t0,
+                      double t_end, int64_t max_steps, int32_t use_graph, double* t_out, int64_t* steps_out,
+                      double* residual_out, int64_t* newton_out) {
+    if (!ctx || !prob || !mesh) return -1;
+    UQK_HIP(ctx, hipSetDevice(ctx->device));
+    if (!ctx->d_run) UQK_HIP(ctx, hipMalloc(&ctx->d_run, sizeof(uqk::RunState)));
+    uqk::RunState init{};
+    init.t = t0;
+    init.active = 1;
+    UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    UQK_HIP(ctx, hipMemcpy(ctx->d_run, &init, sizeof(init), hipMemcpyHostToDevice));
+    const int* active = &ctx->d_run->active;
+    auto enqueue = [&]() -> int {
+        UQK_HIP(ctx, uqk::launch_step_begin(ctx->d_run, t_end, max_steps, ctx->stream));
+        int rc = enqueue_step(ctx, prob, mesh, nt, u, lam, uq, u_scratch, status, iters, active);
+        if (rc) return rc;
+        UQK_HIP(ctx, uqk::launch_step_end(ctx->d_run, ctx->d_scalars, ctx->d_counters, ctx->stream));
+        return 0;
+    };
+    if (use_graph) {
+        char key[512];
+        snprintf(key, sizeof key, "%d %d %.17g %.17g %.17g %.17g %d|%lld %d %d %p %p %p %p %p %p %p|%p %p %p %p %p %p %p|%.17g %lld",
+                 prob->problem, prob->flux, prob->cfl, prob->gamma, prob->gravity, prob->friction, prob->friction_dim,
+                 (long long)mesh->n_cells, mesh->n_edges, mesh->dim, (const void*)mesh->neighbor,
+                 (const void*)mesh->unit_normal, (const void*)mesh->normal, (const void*)mesh->area,
+                 (const void*)mesh->dx_param, (const void*)mesh->bc, (const void*)mesh->boundary_normal,
+                 (const void*)nt, (void*)u, (void*)lam, (void*)uq, (void*)u_scratch, (void*)status, (void*)iters,
+                 t_end, (long long)max_steps);
+        if (!ctx->graph_exec || ctx->graph_key != key) {
+            if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
+            ctx->graph_exec = nullptr;
+            hipGraph_t g = nullptr;
+            UQK_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            ctx->capturing = true;
+            int rc = enqueue();
+            ctx->capturing = false;
+            hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+            if (rc) {
+                if (g) hipGraphDestroy(g);
+                return rc;
+            }
+            UQK_HIP(ctx, ec);
+            hipError_t ei = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            UQK_HIP(ctx, ei);
+            ctx->graph_key = key;
+        }
+    }
+    // replay; every `chunk` steps read the run state back to stop early
+    uqk::RunState rs{};
+    int64_t launched = 0;
+    const int64_t chunk = 64;
+    const int64_t limit = max_steps < 0 ? INT64_MAX : max_steps;
+    for (;;) {
+        const int64_t todo = std::min<int64_t>(chunk, limit - launched);
+        for (int64_t i = 0; i < todo; ++i) {
+            if (use_graph) {
+                UQK_HIP(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+            } else {
+                int rc = enqueue();
+                if (rc) return rc;
+            }
+        }
+        launched += todo;
+        UQK_HIP(ctx, hipMemcpyAsync(&rs, ctx->d_run, sizeof(rs), hipMemcpyDeviceToHost, ctx->stream));
+        UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (!rs.active || launched >= limit || rs.nfail) break;
+    }
+    ctx->timed = false;
+    if (t_out) *t_out = rs.t;
+    if (steps_out) *steps_out = rs.steps;
+    if (residual_out) *residual_out = rs.residual;
+    if (newton_out) *newton_out = rs.newton;
+    return rs.nfail;
 }
 
 int uqk_mean_var(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* lam, double* mv) {

@@ -26,6 +26,18 @@ struct NewtonArgs {
     unsigned long long* nfail;   // [0] failed cells, [1] Newton steps, [2] gradient evaluations; zero at launch
     int wave_doubles;            // LDS doubles per wave
     int prec;                    // UQK_FP64 or UQK_MIXED (Hessian arithmetic)
+    const int* active;           // device flag: 0 skips the launch (graph replays past t_end)
+};
+
+// Device-side state of a multi-step run replayed from a captured graph.
+struct RunState {
+    double t;
+    double residual;
+    long long steps;
+    long long newton;
+    long long gradients;
+    long long nfail;
+    int active;
 };
 
 bool newton_wave_launch(int kind, int M, int Q, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
@@ -65,6 +77,7 @@ struct SweepArgs {
     double* partial;             // per-block partial results
     int npartial;
     double* scalar_out;          // final reduction result
+    const int* active;           // device flag: 0 skips the launch
 };
 
 hipError_t launch_eval_uq(int kind, const SweepArgs& a, hipStream_t st);
@@ -75,5 +88,7 @@ hipError_t launch_residual(const SweepArgs& a, hipStream_t st);
 hipError_t launch_mean_var(int kind, const SweepArgs& a, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, int n, hipStream_t st);
 int sweep_partial_blocks(long ncell);
+hipError_t launch_step_begin(RunState* rs, double t_end, long long max_steps, hipStream_t st);
+hipError_t launch_step_end(RunState* rs, const double* scalars, const unsigned long long* counters, hipStream_t st);
 
 }  // namespace uqk

@@ -73,6 +73,10 @@ SIGNATURES = {
                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
                              C.POINTER(C.c_double)]),
     "uqk_mean_var": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uqk_run_steps": (C.c_int64, [C.c_void_p, C.POINTER(ProblemDesc), C.POINTER(MeshDesc), C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                  C.c_int64, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "uqk_gauss_legendre": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p]),
     "uqk_index_set": (C.c_int32, [C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
     "uqk_basis_tables": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -316,6 +320,15 @@ class GpuClosure:
                                            _p(uq), _p(u_scratch), _p(status), _p(iters), C.byref(dt), C.byref(res)),
                             "uqk_step")
         return nfail, dt.value, res.value
+
+    def run_steps(self, prob, mesh, u, lam, uq, u_scratch, status=None, iters=None, nt_cell=None, t0=0.0,
+                  t_end=1e300, max_steps=-1, use_graph=True):
+        t, steps, res, newton = C.c_double(), C.c_int64(), C.c_double(), C.c_int64()
+        nfail = self._check(lib().uqk_run_steps(self.h, C.byref(prob), C.byref(mesh.desc), _p(nt_cell), _p(u), _p(lam),
+                                                _p(uq), _p(u_scratch), _p(status), _p(iters), t0, t_end, max_steps,
+                                                int(use_graph), C.byref(t), C.byref(steps), C.byref(res),
+                                                C.byref(newton)), "uqk_run_steps")
+        return dict(nfail=nfail, t=t.value, steps=steps.value, residual=res.value, newton_steps=newton.value)
 
     def mean_var(self, lam, mv, nt_cell=None):
         self._check(lib().uqk_mean_var(self.h, lam.shape[0], _p(nt_cell), _p(lam), _p(mv)), "uqk_mean_var")

@@ -56,6 +56,20 @@ class GpuMomentSolver:
                 break
         return self
 
+    def run_device(self, t_end, max_steps=-1, use_graph=True):
+        """The whole loop on the device (uqk_run_steps): the step is replayed
+        from a captured hipGraph with no host synchronisation inside a step."""
+        r = self.cl.run_steps(self.prob, self.dmesh, self.u, self.lam, self.uq, self.u_scratch, self.status,
+                              self.iters, nt_cell=self.nt, t0=self.t, t_end=t_end, max_steps=max_steps,
+                              use_graph=use_graph)
+        if r["nfail"]:
+            raise RuntimeError("closure failed in %d cells at step %d" % (r["nfail"], self.steps + r["steps"]))
+        self.t = r["t"]
+        self.steps += r["steps"]
+        self.residual = r["residual"]
+        self.newton_steps += r["newton_steps"]
+        return self
+
     def mean_var(self):
         mv = native.DeviceArray((self.mesh.n_cells, 2 * self.cl.S))
         self.cl.mean_var(self.lam, mv, nt_cell=self.nt)
