@@ -210,6 +210,31 @@ double uqk_last_closure_ms(const uqk_ctx* ctx);
  * gradient evaluations (line-search trials included) over all cells. */
 int uqk_last_closure_stats(const uqk_ctx* ctx, int64_t* newton_steps, int64_t* gradient_evaluations);
 
+/* ---- whole runs from an input file ------------------------------------- */
+typedef struct {
+    int64_t n_cells, steps, newton_steps;
+    double t, residual, runtime_s, cell_steps_per_s;
+    int32_t n_states, n_moments, n_quad;
+} uqk_run_report;
+
+/* Runs the input file end to end on device 0: Settings from the TOML keys of
+ * settings.cpp:22-323 (plus mesh `type` line|nacaOGrid|quadGrid, problem `ic`,
+ * `flux`, `maxSteps`, moment_system `precision` fp64|mixed), SetupIC and
+ * SetDuals (momentsolver.cpp:463-563), the time loop of MomentSolver::Solve
+ * (momentsolver.cpp:79-181) and, when mean_var_csv is not NULL, the
+ * expectation and variance of every cell (WriteMeanAndVar,
+ * momentsolver.cpp:302-328; one row per cell: S means then S variances).
+ * Returns 0, or -1 with the message in err. */
+int uqk_run_config(const char* toml_path, const char* mean_var_csv, uqk_run_report* report, char* err,
+                   int32_t errlen);
+/* Host-only part of uqk_run_config: parses the file and builds the mesh and
+ * the initial moments without touching a GPU.  info receives (closure, S, M,
+ * Q, d, edges per cell, mesh dim, precision); u and lambda (n_cells x M x S)
+ * and center (n_cells x dim) are written when not NULL.  Returns n_cells, or
+ * -1 with the message in err. */
+int64_t uqk_config_setup(const char* toml_path, int32_t* info, double* u, double* lambda, double* center, char* err,
+                         int32_t errlen);
+
 #ifdef __cplusplus
 }
 #endif

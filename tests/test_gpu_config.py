@@ -1,0 +1,61 @@
+"""Whole runs from input files through uqk_run_config (the C++ host layer:
+Settings, Mesh, SetupIC, SetDuals, the device time loop, WriteMeanAndVar)
+against the oracle's time loop on the same initial state."""
+import os
+
+import numpy as np
+import pytest
+
+from uqcreator_amd import mesh as meshes
+from uqcreator_amd import native
+
+from test_gpu_sweeps import _oracle_closure, _oracle_mesh, _rel_l2
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INPUT = os.path.join(ROOT, "uqcreator_amd", "input")
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(oracle, path, tmp_path, basis, m, kind, prob, clo, t_end, max_steps):
+    csv = str(tmp_path / "mv.csv")
+    print("case: input", path, flush=True)
+    rep = native.run_config(path, csv)
+    inf, u0, lam0, center = native.config_setup(path)
+    assert (inf["M"], inf["Q"]) == (basis.M, basis.Q)
+    np.testing.assert_allclose(center, m.center, atol=1e-12)
+    mv = np.loadtxt(csv, delimiter=",", ndmin=2)
+    c = _oracle_closure(oracle, kind, **clo)
+    res = oracle.run(c, oracle.problem(**prob), _oracle_mesh(oracle, m), basis.phi, basis.phiw, basis.xi, u0, lam0,
+                     t_end, max_steps)
+    assert res["rc"] == 0
+    assert rep["steps"] == res["steps"], (rep, res)
+    mv_o = oracle.mean_var(c, basis.phi, basis.phiw, lam0)
+    return rep, _rel_l2(mv, mv_o, m.area)
+
+
+def test_burgers_input_end_to_end(oracle, tmp_path):
+    rep, err = _run(oracle, os.path.join(INPUT, "burgers1d_step.toml"), tmp_path, native.Basis(1, [5], 12),
+                    meshes.mesh1d(2000, 0.0, 3.0), native.BOUNDED_BARRIER,
+                    dict(kind=native.BURGERS, flux=native.FLUX_LAX_FRIEDRICHS, cfl=0.5),
+                    dict(u_minus=2.9, u_plus=12.1, eps=1e-10), 0.11, -1)
+    print("report", rep, "rel L2 E/Var", err, flush=True)
+    assert rep["steps"] > 1500 and rep["t"] >= 0.11 - 1e-12
+    assert err.max() < 1e-8
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("mixed", 1e-6)])
+def test_euler2d_input_short_run(oracle, tmp_path, precision, tol):
+    text = open(os.path.join(INPUT, "euler2d_naca.toml")).read()
+    for a, b in [("ni = 512", "ni = 64"), ("nj = 128", "nj = 16"), ("maxSteps = 2000", "maxSteps = 10"),
+                 ('precision = "fp64"', 'precision = "%s"' % precision)]:
+        assert a in text
+        text = text.replace(a, b)
+    path = tmp_path / "naca.toml"
+    path.write_text(text)
+    rep, err = _run(oracle, str(path), tmp_path, native.Basis(2, [3], 5), meshes.naca_ogrid(64, 16), native.EULER2D,
+                    dict(kind=native.EULER2D_PROBLEM, flux=native.FLUX_LAX_FRIEDRICHS, cfl=0.4), dict(eps=1e-6),
+                    1e3, 10)
+    print(precision, "report", rep, "rel L2 E/Var", err, flush=True)
+    assert rep["steps"] == 10
+    assert err.max() < tol

@@ -1,0 +1,73 @@
+// config.h -- host classes that keep the reference's interfaces around the
+// C ABI: Settings (TOML keys of settings.cpp:22-323), Mesh (1D line grid,
+// structured quads), Problem (initial conditions), MomentSolver (the time loop
+// of momentsolver.cpp:37-269 driving libuqk on the device).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "toml.h"
+#include "uqk.h"
+
+namespace uqk_host {
+
+struct Settings {
+    // [general]
+    std::string problem;          // Burgers1D | Euler2D | ShallowWater2D
+    std::string output_dir = ".";
+    // [mesh]
+    int dimension = 1;
+    std::string mesh_type;        // line | nacaOGrid | quadGrid
+    long n_cells_1d = 0;
+    double a = 0.0, b = 0.0;
+    int ni = 512, nj = 128;
+    double radius = 20.0, first_height = 2e-3;
+    int nx = 0, ny = 0;
+    double x_min = -1, x_max = 1, y_min = -1, y_max = 1;
+    int boundary = UQK_BC_DIRICHLET;
+    // [problem]
+    std::vector<double> sigma;
+    double cfl = 0.5, t_end = 0.0, min_residual = -1.0, gamma = 1.4;
+    int flux = UQK_FLUX_REFERENCE;
+    long max_steps = -1;
+    double friction = 0.0;
+    int friction_dim = -1;
+    std::string ic;
+    // [moment_system]
+    int closure = UQK_BOUNDED_BARRIER;
+    bool max_degree = false;
+    std::vector<int> degrees;
+    int quad_points = 0;
+    int max_iterations = 1000;
+    double epsilon = 5e-5;
+    double u_minus = 3.0, u_plus = 12.0;
+    double barrier = 0.01;
+    int precision = UQK_FP64;
+    bool du_exact = true;
+    bool safe_line_search = false;
+    // derived
+    int dim_xi = 0, n_states = 0;
+
+    static Settings from_toml(const Toml& t);
+};
+
+struct Mesh {
+    long n_cells = 0;
+    int n_edges = 0, dim = 0;
+    std::vector<int32_t> nbr, bc;
+    std::vector<double> unit_normal, normal, area, dx_param, bnormal, center;
+
+    static Mesh line(long n, double a, double b);
+    static Mesh quad(const std::vector<double>& X, const std::vector<double>& Y, int ni_nodes, int nj_nodes,
+                     bool periodic_i, int bc_j0, int bc_j1, int bc_i0, int bc_i1);
+    static Mesh quad_grid(int nx, int ny, double x0, double x1, double y0, double y1, int bc);
+    static Mesh naca_ogrid(int ni, int nj, double radius, double first_height);
+    static Mesh from_settings(const Settings& s);
+};
+
+// IC(x, xi) -> S values (Problem::IC, burgers.cpp:48-86, euler2d.cpp:136-206,
+// shallowwater2d.cpp:77-107 and the config dam break)
+void initial_state(const Settings& s, const double* x, const double* xi, double* out);
+
+}  // namespace uqk_host

@@ -42,6 +42,14 @@ class ProblemDesc(C.Structure):
     ]
 
 
+class RunReport(C.Structure):
+    _fields_ = [
+        ("n_cells", C.c_int64), ("steps", C.c_int64), ("newton_steps", C.c_int64), ("t", C.c_double),
+        ("residual", C.c_double), ("runtime_s", C.c_double), ("cell_steps_per_s", C.c_double),
+        ("n_states", C.c_int32), ("n_moments", C.c_int32), ("n_quad", C.c_int32),
+    ]
+
+
 class MeshDesc(C.Structure):
     _fields_ = [
         ("n_cells", C.c_int64), ("n_edges", C.c_int32), ("dim", C.c_int32), ("neighbor", C.c_void_p),
@@ -92,6 +100,9 @@ SIGNATURES = {
     "uqk_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32]),
     "uqk_device_synchronize": (C.c_int, []),
     "uqk_device_count": (C.c_int32, []),
+    "uqk_run_config": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(RunReport), C.c_char_p, C.c_int32]),
+    "uqk_config_setup": (C.c_int64, [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p,
+                                     C.c_int32]),
 }
 
 _lib = None
@@ -377,3 +388,34 @@ def synth_closure_euler1d(ncell, basis, seed=20261202, device=True):
     if nf != 0:
         raise RuntimeError("workload generation failed (%d)" % nf)
     return tuple(out)
+
+
+def config_setup(path):
+    """Host-only part of a run from an input file: (info, u, lambda, center)
+    with info = dict(closure, S, M, Q, d, edges, dim, precision)."""
+    err = C.create_string_buffer(512)
+    info = np.zeros(8, np.int32)
+    n = lib().uqk_config_setup(path.encode(), _p(info), None, None, None, err, 512)
+    if n < 0:
+        raise RuntimeError(err.value.decode())
+    keys = ["closure", "S", "M", "Q", "d", "edges", "dim", "precision"]
+    inf = dict(zip(keys, (int(v) for v in info)))
+    u = np.zeros((n, inf["M"], inf["S"]))
+    lam = np.zeros_like(u)
+    center = np.zeros((n, inf["dim"]))
+    n2 = lib().uqk_config_setup(path.encode(), None, _p(u), _p(lam), _p(center), err, 512)
+    if n2 != n:
+        raise RuntimeError(err.value.decode())
+    return inf, u, lam, center
+
+
+def run_config(path, mean_var_csv=None):
+    """Runs an input file end to end on the GPU (uqk_run_config); returns the
+    run report as a dict."""
+    err = C.create_string_buffer(1024)
+    rep = RunReport()
+    rc = lib().uqk_run_config(path.encode(), mean_var_csv.encode() if mean_var_csv else None, C.byref(rep), err,
+                              1024)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return {k: getattr(rep, k) for k, _ in RunReport._fields_}
