@@ -210,6 +210,51 @@ double uqk_last_closure_ms(const uqk_ctx* ctx);
  * gradient evaluations (line-search trials included) over all cells. */
 int uqk_last_closure_stats(const uqk_ctx* ctx, int64_t* newton_steps, int64_t* gradient_evaluations);
 
+/* ---- learned warm start (no counterpart in the reference) --------------- */
+/* A network that maps a cell's moments to a starting dual for the Newton
+ * closure of an Euler context (UQK_EULER1D / UQK_EULER2D).  Inputs are the
+ * moments scaled by the cell's mean state (rho0, rho0 sqrt(T0), rho0 T0),
+ * outputs the correction to the plain start (DS(u_0), 0, ...) in the same
+ * scaling; four hidden layers of `hidden` units with GELU, BF16 weights on
+ * MFMA with FP32 accumulation, trained on the GPU with Adam.  All array
+ * arguments are device pointers unless named host_. */
+typedef struct uqk_mlp uqk_mlp;
+int uqk_mlp_create(uqk_ctx* ctx, int32_t hidden, uint64_t seed, uqk_mlp** out);
+int uqk_mlp_destroy(uqk_mlp* net);
+/* dims[0..5] = inputs, hidden x 4, outputs (each S*M rounded up to 32);
+ * returns the number of layers (5). */
+int uqk_mlp_shape(const uqk_mlp* net, int32_t* dims, int64_t* n_params);
+/* FP32 parameters [W_1 (out x in, row-major), b_1, ..., W_5, b_5] and the
+ * normalisation [mu_x, sd_x, mu_t, sd_t] (host buffers; either may be NULL). */
+int uqk_mlp_get_params(uqk_mlp* net, float* host_params, float* host_norm);
+int uqk_mlp_set_params(uqk_mlp* net, const float* host_params, const float* host_norm);
+/* Per-feature mean and std of inputs and targets from (u, lambda) pairs. */
+int uqk_mlp_fit_normalization(uqk_mlp* net, int64_t n_pairs, const double* u, const double* lambda);
+/* Features x (n x inputs) and, when lambda is not NULL, targets t
+ * (n x outputs), raw or standardised. */
+int uqk_mlp_features(uqk_mlp* net, int64_t n, const double* u, const double* lambda, int32_t standardise, float* x,
+                     float* t);
+/* Network output y (rows x outputs) for standardised features x. */
+int uqk_mlp_forward(uqk_mlp* net, int64_t rows, const float* x, float* y);
+/* Mean squared error over the first n_real outputs and its gradient (same
+ * layout as the parameters) for one batch (rows a multiple of 32). */
+int uqk_mlp_loss_grad(uqk_mlp* net, int64_t rows, const float* x, const float* t, int32_t n_real, double* host_loss,
+                      float* grad);
+/* `steps` Adam steps on minibatches of `batch` pairs (a multiple of 64) drawn
+ * with replacement from the n_pairs (u, lambda) pairs; fits the
+ * normalisation first if it was never set.  host_mean_loss: mean batch loss. */
+int uqk_mlp_train(uqk_mlp* net, int64_t n_pairs, const double* u, const double* lambda, int32_t batch, int32_t steps,
+                  double lr, uint64_t seed, double* host_mean_loss);
+/* lambda_out (n x M x S) = the network's start for moments u. */
+int uqk_mlp_predict(uqk_mlp* net, int64_t n, const double* u, double* lambda_out);
+/* Per cell keep in lambda whichever of lambda and lambda_alt has the smaller
+ * ||grad L||_2 (non-finite never wins).  Returns the number of cells that
+ * took lambda_alt. */
+int64_t uqk_warm_start_select(uqk_ctx* ctx, int64_t n, const double* u, double* lambda, const double* lambda_alt);
+/* Attach (or detach with NULL) a network: every later closure of ctx with
+ * uniform orders first predicts a start and keeps the better one per cell. */
+int uqk_set_warm_start(uqk_ctx* ctx, uqk_mlp* net);
+
 /* ---- whole runs from an input file ------------------------------------- */
 typedef struct {
     int64_t n_cells, steps, newton_steps;

@@ -59,6 +59,12 @@ def lib():
         L = _lib
         L.orc_legendre.restype = C.c_double
         L.orc_legendre.argtypes = [C.c_int, C.c_double]
+        L.orc_mlp_param_count.restype = C.c_long
+        L.orc_gelu.restype = C.c_double
+        L.orc_gelu.argtypes = [C.c_double, C.c_int]
+        L.orc_bf16_round.restype = C.c_double
+        L.orc_bf16_round.argtypes = [C.c_double]
+        L.orc_mlp_loss_grad.restype = C.c_double
         L.orc_compute_dt.restype = C.c_double
         L.orc_compute_dt.argtypes = [C.POINTER(Problem), C.c_int, C.c_int, C.c_void_p, C.c_double]
     return _lib
@@ -275,3 +281,40 @@ def rel_error(sol, ref, area, lnorm=2, mask=None):
     lib().orc_rel_error(nf, C.c_long(ncell), _p(np.ascontiguousarray(sol)), _p(np.ascontiguousarray(ref)),
                         _p(np.ascontiguousarray(area)), _p(m), lnorm, _p(err))
     return err
+
+
+def _mlp_dims(dims):
+    return np.ascontiguousarray(dims, dtype=np.int32)
+
+
+def mlp_param_count(dims):
+    d = _mlp_dims(dims)
+    return lib().orc_mlp_param_count(len(d) - 1, _p(d))
+
+
+def gelu(z, derivative=False):
+    return np.vectorize(lambda v: lib().orc_gelu(C.c_double(v), int(derivative)))(z)
+
+
+def bf16_round(v):
+    return np.vectorize(lambda a: lib().orc_bf16_round(C.c_double(a)))(v)
+
+
+def mlp_forward(dims, params, x, emulate_bf16=False):
+    d = _mlp_dims(dims)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros((x.shape[0], d[-1]))
+    lib().orc_mlp_forward(len(d) - 1, _p(d), _p(np.ascontiguousarray(params, dtype=np.float64)), C.c_long(x.shape[0]),
+                          _p(x), _p(y), int(emulate_bf16))
+    return y
+
+
+def mlp_loss_grad(dims, params, x, t, n_real=None, emulate_bf16=False):
+    d = _mlp_dims(dims)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    g = np.zeros(mlp_param_count(dims))
+    loss = lib().orc_mlp_loss_grad(len(d) - 1, _p(d), _p(np.ascontiguousarray(params, dtype=np.float64)),
+                                   C.c_long(x.shape[0]), _p(x), _p(t), int(d[-1] if n_real is None else n_real),
+                                   _p(g), int(emulate_bf16))
+    return loss, g

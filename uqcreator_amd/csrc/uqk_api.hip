@@ -13,35 +13,7 @@
 
 #include "uqk.h"
 #include "uqk_internal.h"
-
-struct uqk_ctx {
-    int device = 0;
-    int kind = 0;
-    int S = 0, M = 0, Q = 0, d = 0;
-    int precision = UQK_FP64;
-    uqk::ClosureParams c{};
-    double* d_phi = nullptr;
-    double* d_phiw = nullptr;
-    double* d_xi = nullptr;
-    unsigned long long* d_counters = nullptr;  // [0] queue head, [1] failed cells, [2] Newton steps, [3] gradients
-    double* d_partial = nullptr;               // per-block partials
-    double* d_scalars = nullptr;               // [0] dt, [1] residual
-    hipStream_t own_stream = nullptr;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double last_closure_ms = 0.0;
-    unsigned long long last_stats[3] = {0, 0, 0};
-    int kernel = UQK_KERNEL_AUTO;
-    int last_kernel = UQK_KERNEL_AUTO;
-    bool timed = false;  // ev0/ev1 bracket the last closure kernel
-    bool capturing = false;  // enqueueing into a graph: no events, no syncs
-    uqk::RunState* d_run = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    std::string graph_key;
-    size_t max_lds = 0;
-    int num_cu = 0;
-    std::string err;
-};
+#include "uqk_ctx.h"
 
 namespace {
 
@@ -110,8 +82,12 @@ int expected_states(int kind) {
 int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, double* lam, int32_t* status,
                    int32_t* iters, const int* active = nullptr) {
     ctx->timed = false;
-    UQK_HIP(ctx, uqk::launch_fill_u64(ctx->d_counters, 4, ctx->stream));
+    UQK_HIP(ctx, uqk::launch_fill_u64(ctx->d_counters, 4, ctx->stream));  // [4..7] belong to the warm start
     if (n <= 0) return 0;
+    if (ctx->warm && !nt) {
+        int rc = uqk::enqueue_warm_start(ctx, n, u, lam, active);
+        if (rc) return rc;
+    }
     uqk::NewtonArgs a{};
     a.c = ctx->c;
     a.M = ctx->M;
@@ -276,7 +252,7 @@ int uqk_create(const uqk_desc* desc, uqk_ctx** out) {
         UQK_HIP(ctx, hipMalloc(&ctx->d_xi, xb));
         UQK_HIP(ctx, hipMemcpy(ctx->d_xi, desc->xi, xb, hipMemcpyHostToDevice));
     }
-    UQK_HIP(ctx, hipMalloc(&ctx->d_counters, 4 * sizeof(unsigned long long)));
+    UQK_HIP(ctx, hipMalloc(&ctx->d_counters, 8 * sizeof(unsigned long long)));
     UQK_HIP(ctx, hipMalloc(&ctx->d_partial, sizeof(double) * uqk::sweep_partial_blocks(0)));
     UQK_HIP(ctx, hipMalloc(&ctx->d_scalars, 4 * sizeof(double)));
     UQK_HIP(ctx, hipStreamCreate(&ctx->own_stream));

@@ -1,0 +1,53 @@
+// uqk_ctx.h -- the context behind the opaque uqk_ctx handle of the C ABI:
+// basis tables in HBM, closure parameters, stream, events, reduction buffers
+// and the captured step graph.  Shared by the translation units that
+// implement the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "uqk.h"
+#include "uqk_internal.h"
+
+struct uqk_mlp;
+
+struct uqk_ctx {
+    int device = 0;
+    int kind = 0;
+    int S = 0, M = 0, Q = 0, d = 0;
+    int precision = UQK_FP64;
+    uqk::ClosureParams c{};
+    double* d_phi = nullptr;
+    double* d_phiw = nullptr;
+    double* d_xi = nullptr;
+    unsigned long long* d_counters = nullptr;  // [0] queue head, [1] failed cells, [2] Newton steps, [3] gradients,
+                                               // [4] cells that took the learned warm start
+    double* d_partial = nullptr;               // per-block partials
+    double* d_scalars = nullptr;               // [0] dt, [1] residual
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_closure_ms = 0.0;
+    unsigned long long last_stats[3] = {0, 0, 0};
+    int kernel = UQK_KERNEL_AUTO;
+    int last_kernel = UQK_KERNEL_AUTO;
+    bool timed = false;  // ev0/ev1 bracket the last closure kernel
+    bool capturing = false;  // enqueueing into a graph: no events, no syncs
+    uqk::RunState* d_run = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::string graph_key;
+    size_t max_lds = 0;
+    int num_cu = 0;
+    uqk_mlp* warm = nullptr;  // learned warm start run before each closure (not owned)
+    std::string err;
+};
+
+namespace uqk {
+// learned warm start (mlp.hip): predict with the network attached to ctx and
+// keep per cell the start with the smaller ||grad L||; enqueue only.
+int enqueue_warm_start(uqk_ctx* ctx, long ncell, const double* u, double* lambda, const int* active);
+int enqueue_warm_select(uqk_ctx* ctx, long ncell, const double* u, double* lambda, const double* lambda_alt,
+                        const int* active);
+}  // namespace uqk

@@ -100,6 +100,22 @@ SIGNATURES = {
     "uqk_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32]),
     "uqk_device_synchronize": (C.c_int, []),
     "uqk_device_count": (C.c_int32, []),
+    "uqk_mlp_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "uqk_mlp_destroy": (C.c_int, [C.c_void_p]),
+    "uqk_mlp_shape": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]),
+    "uqk_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uqk_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uqk_mlp_fit_normalization": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "uqk_mlp_features": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
+                                   C.c_void_p]),
+    "uqk_mlp_forward": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "uqk_mlp_loss_grad": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32,
+                                    C.POINTER(C.c_double), C.c_void_p]),
+    "uqk_mlp_train": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double,
+                                C.c_uint64, C.POINTER(C.c_double)]),
+    "uqk_mlp_predict": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "uqk_warm_start_select": (C.c_int64, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uqk_set_warm_start": (C.c_int, [C.c_void_p, C.c_void_p]),
     "uqk_run_config": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(RunReport), C.c_char_p, C.c_int32]),
     "uqk_config_setup": (C.c_int64, [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p,
                                      C.c_int32]),
@@ -281,6 +297,9 @@ class GpuClosure:
             raise RuntimeError("%s: %s (%d)" % (what, lib().uqk_last_error(self.h).decode(), rc))
         return rc
 
+    def error(self):
+        return lib().uqk_last_error(self.h).decode()
+
     def set_kernel(self, kernel):
         """KERNEL_AUTO, KERNEL_GENERIC or KERNEL_WAVE."""
         self._check(lib().uqk_set_closure_kernel(self.h, kernel), "uqk_set_closure_kernel")
@@ -419,3 +438,90 @@ def run_config(path, mean_var_csv=None):
     if rc != 0:
         raise RuntimeError(err.value.decode())
     return {k: getattr(rep, k) for k, _ in RunReport._fields_}
+
+
+class WarmStartNet:
+    """Learned warm start of an Euler closure context (uqk_mlp_* entry points):
+    four hidden layers of `hidden` GELU units, BF16 weights on MFMA, trained on
+    the GPU with Adam on (u, lambda) pairs of device arrays."""
+
+    def __init__(self, closure, hidden=256, seed=1):
+        self.closure = closure
+        h = C.c_void_p()
+        if lib().uqk_mlp_create(closure.h, hidden, seed, C.byref(h)) != 0:
+            raise RuntimeError("uqk_mlp_create: " + closure.error())
+        self.h = h
+        dims = np.zeros(6, np.int32)
+        npar = C.c_int64()
+        lib().uqk_mlp_shape(self.h, _p(dims), C.byref(npar))
+        self.dims = [int(v) for v in dims]
+        self.n_params = npar.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().uqk_mlp_destroy(self.h)
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError("%s: %s" % (what, self.closure.error()))
+
+    def get_params(self):
+        p = np.zeros(self.n_params, np.float32)
+        nrm = np.zeros(2 * self.dims[0] + 2 * self.dims[-1], np.float32)
+        self._check(lib().uqk_mlp_get_params(self.h, _p(p), _p(nrm)), "uqk_mlp_get_params")
+        return p, nrm
+
+    def set_params(self, params=None, norm=None):
+        pp = None if params is None else np.ascontiguousarray(params, np.float32)
+        nn = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        self._check(lib().uqk_mlp_set_params(self.h, _p(pp), _p(nn)), "uqk_mlp_set_params")
+
+    def fit_normalization(self, u, lam):
+        self._check(lib().uqk_mlp_fit_normalization(self.h, u.shape[0], _p(u), _p(lam)), "fit_normalization")
+
+    def features(self, u, lam=None, standardise=False):
+        n = u.shape[0]
+        x = DeviceArray((n, self.dims[0]), np.float32)
+        t = DeviceArray((n, self.dims[-1]), np.float32) if lam is not None else None
+        self._check(lib().uqk_mlp_features(self.h, n, _p(u), _p(lam), int(standardise), _p(x), _p(t)),
+                    "uqk_mlp_features")
+        return x, t
+
+    def forward(self, x):
+        y = DeviceArray((x.shape[0], self.dims[-1]), np.float32)
+        self._check(lib().uqk_mlp_forward(self.h, x.shape[0], _p(x), _p(y)), "uqk_mlp_forward")
+        return y
+
+    def loss_grad(self, x, t, n_real):
+        g = DeviceArray(self.n_params, np.float32)
+        loss = C.c_double()
+        self._check(lib().uqk_mlp_loss_grad(self.h, x.shape[0], _p(x), _p(t), n_real, C.byref(loss), _p(g)),
+                    "uqk_mlp_loss_grad")
+        return loss.value, g
+
+    def train(self, u, lam, batch=1024, steps=1000, lr=1e-3, seed=7):
+        loss = C.c_double()
+        self._check(lib().uqk_mlp_train(self.h, u.shape[0], _p(u), _p(lam), batch, steps, lr, seed,
+                                        C.byref(loss)), "uqk_mlp_train")
+        return loss.value
+
+    def predict(self, u):
+        out = DeviceArray(u.shape, np.float64)
+        self._check(lib().uqk_mlp_predict(self.h, u.shape[0], _p(u), _p(out)), "uqk_mlp_predict")
+        return out
+
+    def attach(self):
+        self._check(lib().uqk_set_warm_start(self.closure.h, self.h), "uqk_set_warm_start")
+
+    def detach(self):
+        lib().uqk_set_warm_start(self.closure.h, None)
+
+
+def warm_start_select(closure, u, lam, lam_alt):
+    n = lib().uqk_warm_start_select(closure.h, u.shape[0], _p(u), _p(lam), _p(lam_alt))
+    if n < 0:
+        raise RuntimeError("uqk_warm_start_select: " + closure.error())
+    return n
