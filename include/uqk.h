@@ -210,6 +210,22 @@ double uqk_last_closure_ms(const uqk_ctx* ctx);
  * gradient evaluations (line-search trials included) over all cells. */
 int uqk_last_closure_stats(const uqk_ctx* ctx, int64_t* newton_steps, int64_t* gradient_evaluations);
 
+/* ---- launch shapes and the autotuner (no counterpart in the reference) --- */
+/* Launch shape of the specialised closure kernel: wavefronts (cells in
+ * flight) per workgroup (1, 2, 4; 0 = as many as LDS allows) and persistent
+ * workgroups per CU (0 = every resident slot). */
+int uqk_set_launch_shape(uqk_ctx* ctx, int32_t waves, int32_t blocks_per_cu);
+int uqk_get_launch_shape(const uqk_ctx* ctx, int32_t* waves, int32_t* blocks_per_cu);
+/* Times the specialised closure kernel on n cells (device u and starting
+ * lambda; lambda is not modified) for every launch shape the LDS and the
+ * occupancy allow, keeps the fastest for ctx, and appends it as one JSON line
+ * {closure, n_states, n_moments, n_quad, precision, waves, blocks_per_cu,
+ * ms, cells} to json_path when not NULL.  Returns the number of shapes timed. */
+int uqk_autotune(uqk_ctx* ctx, int64_t n, const double* u, const double* lambda, const char* json_path);
+/* Applies the last line of json_path that matches ctx's closure, sizes and
+ * precision.  Returns 1 if one matched, 0 if none. */
+int uqk_load_tuning(uqk_ctx* ctx, const char* json_path);
+
 /* ---- learned warm start (no counterpart in the reference) --------------- */
 /* A network that maps a cell's moments to a starting dual for the Newton
  * closure of an Euler context (UQK_EULER1D / UQK_EULER2D).  Inputs are the

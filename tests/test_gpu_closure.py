@@ -216,3 +216,29 @@ def test_mixed_precision_closure(oracle, kind, d, deg, q):
     assert (np.abs(lam - lam_star) / scale).max() < 1e-7
     # the FP32 Hessian changes the Newton path only slightly
     assert it.mean() <= it_o.mean() + 1.0
+
+
+def test_autotuned_launch_shape_gives_identical_duals(tmp_path):
+    """The autotuner times every launch shape on a sample and keeps the
+    fastest; a cell's solve does not depend on the shape, so the duals are
+    bitwise equal to the default shape's; the JSON line reloads."""
+    b = workloads.closure_basis()
+    u, lam_star, lam0 = workloads.closure_euler1d(8192, basis=b, device=True)
+    cl = native.GpuClosure(native.EULER1D, b, epsilon=1e-10, safe_line_search=1)
+    ref = native.DeviceArray.from_host(lam0.download())
+    assert cl.solve(u, ref) == 0
+    path = str(tmp_path / "tuning.jsonl")
+    print("case: autotune C2 8192 cells", flush=True)
+    n = cl.autotune(u, lam0, path)
+    shape = cl.launch_shape()
+    print("shapes timed", n, "best", shape, open(path).read(), flush=True)
+    assert n >= 1 and shape[0] in (1, 2, 4) and shape[1] >= 1
+    out = native.DeviceArray.from_host(lam0.download())
+    assert cl.solve(u, out) == 0
+    np.testing.assert_array_equal(out.download(), ref.download())
+    cl2 = native.GpuClosure(native.EULER1D, b, epsilon=1e-10, safe_line_search=1)
+    assert cl2.load_tuning(path) == 1 and cl2.launch_shape() == shape
+    cl2.set_launch_shape(1, 1)
+    out2 = native.DeviceArray.from_host(lam0.download())
+    assert cl2.solve(u, out2) == 0
+    np.testing.assert_array_equal(out2.download(), ref.download())

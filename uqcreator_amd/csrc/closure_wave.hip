@@ -569,47 +569,94 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
 // dispatch over the compiled shapes
 // ---------------------------------------------------------------------------
 template <int KIND, int M, int Q, int PREC>
-static void wave_go(const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st, hipError_t* err) {
+static size_t wave_lds(int waves, size_t* table_bytes) {
     using W = WaveShape<KIND, M, Q>;
     const size_t mop = PREC == 1 ? sizeof(float) * 2 * ((M + 15) / 16) * ((Q + 3) / 4) * 64 : 0;
-    const size_t tb = sizeof(double) * W::TABLE_DOUBLES + mop, wbytes = sizeof(double) * W::WAVE_DOUBLES;
+    const size_t tb = sizeof(double) * W::TABLE_DOUBLES + mop;
+    if (table_bytes) *table_bytes = tb;
+    return tb + (size_t)waves * sizeof(double) * W::WAVE_DOUBLES;
+}
+
+template <int KIND, int M, int Q, int PREC>
+static int wave_max_waves(int max_lds) {
     int waves = 4;
-    while (waves > 1 && tb + waves * wbytes > (size_t)max_lds) --waves;
-    const int threads = 64 * waves;
-    const size_t shmem = tb + waves * wbytes;
+    while (waves > 1 && wave_lds<KIND, M, Q, PREC>(waves, nullptr) > (size_t)max_lds) --waves;
+    return waves;
+}
+
+template <int KIND, int M, int Q, int PREC>
+static int wave_occupancy(int waves) {
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, newton_wave_kernel<KIND, M, Q, PREC>, threads, shmem) !=
-            hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, newton_wave_kernel<KIND, M, Q, PREC>, 64 * waves,
+                                                     wave_lds<KIND, M, Q, PREC>(waves, nullptr)) != hipSuccess ||
         occ <= 0)
         occ = 1;
+    return occ;
+}
+
+template <int KIND, int M, int Q, int PREC>
+static void wave_go(const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st, hipError_t* err,
+                    const LaunchShape& shape) {
+    const int wmax = wave_max_waves<KIND, M, Q, PREC>(max_lds);
+    int waves = shape.waves > 0 ? (shape.waves < wmax ? shape.waves : wmax) : wmax;
+    const int threads = 64 * waves;
+    const size_t shmem = wave_lds<KIND, M, Q, PREC>(waves, nullptr);
+    const int occ = wave_occupancy<KIND, M, Q, PREC>(waves);
+    const int per_cu = shape.blocks_per_cu > 0 && shape.blocks_per_cu < occ ? shape.blocks_per_cu : occ;
     long want = (a.ncell + waves - 1) / waves;
-    int blocks = (int)(want < (long)occ * num_cu ? want : (long)occ * num_cu);
+    int blocks = (int)(want < (long)per_cu * num_cu ? want : (long)per_cu * num_cu);
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL((newton_wave_kernel<KIND, M, Q, PREC>), dim3(blocks), dim3(threads), shmem, st, a);
     *err = hipGetLastError();
 }
 
 template <int KIND, int M, int Q>
-static bool wave_try(int kind, int m, int q, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
-                     hipError_t* err) {
+static bool wave_fits(int kind, int m, int q, int max_lds) {
     if (kind != KIND || m != M || q != Q) return false;
     using W = WaveShape<KIND, M, Q>;
-    if (sizeof(double) * (W::TABLE_DOUBLES + W::WAVE_DOUBLES) > (size_t)max_lds) return false;
+    return sizeof(double) * (W::TABLE_DOUBLES + W::WAVE_DOUBLES) <= (size_t)max_lds;
+}
+
+template <int KIND, int M, int Q>
+static bool wave_try(int kind, int m, int q, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
+                     hipError_t* err, const LaunchShape& shape) {
+    if (!wave_fits<KIND, M, Q>(kind, m, q, max_lds)) return false;
     if (a.prec == UQK_FP64)
-        wave_go<KIND, M, Q, 0>(a, max_lds, num_cu, st, err);
+        wave_go<KIND, M, Q, 0>(a, max_lds, num_cu, st, err, shape);
     else
-        wave_go<KIND, M, Q, 1>(a, max_lds, num_cu, st, err);
+        wave_go<KIND, M, Q, 1>(a, max_lds, num_cu, st, err, shape);
+    return true;
+}
+
+template <int KIND, int M, int Q>
+static bool wave_limits(int kind, int m, int q, int prec, int max_lds, int waves, int* max_waves, int* occupancy) {
+    if (!wave_fits<KIND, M, Q>(kind, m, q, max_lds)) return false;
+    if (prec == UQK_FP64) {
+        *max_waves = wave_max_waves<KIND, M, Q, 0>(max_lds);
+        *occupancy = wave_occupancy<KIND, M, Q, 0>(waves < *max_waves ? waves : *max_waves);
+    } else {
+        *max_waves = wave_max_waves<KIND, M, Q, 1>(max_lds);
+        *occupancy = wave_occupancy<KIND, M, Q, 1>(waves < *max_waves ? waves : *max_waves);
+    }
     return true;
 }
 
 bool newton_wave_launch(int kind, int M, int Q, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
-                        hipError_t* err) {
+                        hipError_t* err, const LaunchShape& shape) {
     if (a.nt_cell) return false;  // per-cell orders use the generic kernel
-    return wave_try<UQK_BOUNDED_BARRIER, 6, 12>(kind, M, Q, a, max_lds, num_cu, st, err) ||
-           wave_try<UQK_EULER1D, 15, 49>(kind, M, Q, a, max_lds, num_cu, st, err) ||
-           wave_try<UQK_EULER2D, 10, 25>(kind, M, Q, a, max_lds, num_cu, st, err) ||
-           wave_try<UQK_SHALLOW_WATER2D, 10, 25>(kind, M, Q, a, max_lds, num_cu, st, err) ||
-           wave_try<UQK_SHALLOW_WATER2D_BARRIER, 10, 25>(kind, M, Q, a, max_lds, num_cu, st, err);
+    return wave_try<UQK_BOUNDED_BARRIER, 6, 12>(kind, M, Q, a, max_lds, num_cu, st, err, shape) ||
+           wave_try<UQK_EULER1D, 15, 49>(kind, M, Q, a, max_lds, num_cu, st, err, shape) ||
+           wave_try<UQK_EULER2D, 10, 25>(kind, M, Q, a, max_lds, num_cu, st, err, shape) ||
+           wave_try<UQK_SHALLOW_WATER2D, 10, 25>(kind, M, Q, a, max_lds, num_cu, st, err, shape) ||
+           wave_try<UQK_SHALLOW_WATER2D_BARRIER, 10, 25>(kind, M, Q, a, max_lds, num_cu, st, err, shape);
+}
+
+bool newton_wave_limits(int kind, int M, int Q, int prec, int max_lds, int waves, int* max_waves, int* occupancy) {
+    return wave_limits<UQK_BOUNDED_BARRIER, 6, 12>(kind, M, Q, prec, max_lds, waves, max_waves, occupancy) ||
+           wave_limits<UQK_EULER1D, 15, 49>(kind, M, Q, prec, max_lds, waves, max_waves, occupancy) ||
+           wave_limits<UQK_EULER2D, 10, 25>(kind, M, Q, prec, max_lds, waves, max_waves, occupancy) ||
+           wave_limits<UQK_SHALLOW_WATER2D, 10, 25>(kind, M, Q, prec, max_lds, waves, max_waves, occupancy) ||
+           wave_limits<UQK_SHALLOW_WATER2D_BARRIER, 10, 25>(kind, M, Q, prec, max_lds, waves, max_waves, occupancy);
 }
 
 }  // namespace uqk

@@ -108,7 +108,8 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     if (ctx->kernel != UQK_KERNEL_GENERIC) {
         hipError_t e = hipSuccess;
         if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-        if (uqk::newton_wave_launch(ctx->kind, ctx->M, ctx->Q, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream, &e)) {
+        if (uqk::newton_wave_launch(ctx->kind, ctx->M, ctx->Q, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream, &e,
+                                    ctx->shape)) {
             UQK_HIP(ctx, e);
             if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
             ctx->last_kernel = UQK_KERNEL_WAVE;
@@ -531,6 +532,100 @@ int uqk_last_closure_stats(const uqk_ctx* ctx, int64_t* newton_steps, int64_t* g
     if (newton_steps) *newton_steps = (int64_t)ctx->last_stats[1];
     if (gradient_evaluations) *gradient_evaluations = (int64_t)ctx->last_stats[2];
     return 0;
+}
+
+int uqk_set_launch_shape(uqk_ctx* ctx, int32_t waves, int32_t blocks_per_cu) {
+    if (!ctx) return -1;
+    if (waves != 0 && waves != 1 && waves != 2 && waves != 4) return fail(ctx, -1, "waves per workgroup: 0, 1, 2 or 4");
+    if (blocks_per_cu < 0) return fail(ctx, -1, "workgroups per CU must be >= 0");
+    ctx->shape.waves = waves;
+    ctx->shape.blocks_per_cu = blocks_per_cu;
+    return 0;
+}
+
+int uqk_get_launch_shape(const uqk_ctx* ctx, int32_t* waves, int32_t* blocks_per_cu) {
+    if (!ctx) return -1;
+    if (waves) *waves = ctx->shape.waves;
+    if (blocks_per_cu) *blocks_per_cu = ctx->shape.blocks_per_cu;
+    return 0;
+}
+
+int uqk_autotune(uqk_ctx* ctx, int64_t n, const double* u, const double* lambda, const char* json_path) {
+    if (!ctx || n <= 0 || !u || !lambda) return -1;
+    int wmax = 0, occ = 0;
+    if (!uqk::newton_wave_limits(ctx->kind, ctx->M, ctx->Q, ctx->precision, (int)ctx->max_lds, 4, &wmax, &occ))
+        return fail(ctx, -3, "uqk_autotune: no specialised closure kernel for this shape");
+    const size_t bytes = sizeof(double) * (size_t)n * ctx->S * ctx->M;
+    double* scratch = nullptr;
+    UQK_HIP(ctx, hipMalloc(&scratch, bytes));
+    const uqk::LaunchShape keep = ctx->shape;
+    const int kernel_keep = ctx->kernel;
+    ctx->kernel = UQK_KERNEL_WAVE;
+    double best_ms = -1.0;
+    uqk::LaunchShape best = keep;
+    int timed = 0, rc = 0;
+    for (int waves = 1; waves <= wmax && rc == 0; waves *= 2) {
+        int wm = 0, oc = 0;
+        uqk::newton_wave_limits(ctx->kind, ctx->M, ctx->Q, ctx->precision, (int)ctx->max_lds, waves, &wm, &oc);
+        for (int per_cu = 1; per_cu <= oc && rc == 0; ++per_cu) {
+            ctx->shape.waves = waves;
+            ctx->shape.blocks_per_cu = per_cu;
+            double ms_min = 1e300;
+            for (int rep = 0; rep < 2 && rc == 0; ++rep) {  // first run warms the caches
+                if (hipMemcpyAsync(scratch, lambda, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) {
+                    rc = fail(ctx, -2, "uqk_autotune: copy failed");
+                    break;
+                }
+                rc = launch_closure(ctx, n, nullptr, u, scratch, nullptr, nullptr);
+                if (rc) break;
+                if (finish_closure(ctx) < 0) rc = -2;
+                if (rep == 1) ms_min = ctx->last_closure_ms;
+            }
+            if (rc) break;
+            ++timed;
+            if (best_ms < 0 || ms_min < best_ms) {
+                best_ms = ms_min;
+                best = ctx->shape;
+            }
+        }
+    }
+    hipFree(scratch);
+    ctx->kernel = kernel_keep;
+    ctx->shape = rc ? keep : best;
+    if (rc) return rc;
+    if (json_path) {
+        FILE* f = std::fopen(json_path, "a");
+        if (!f) return fail(ctx, -1, std::string("uqk_autotune: cannot write ") + json_path);
+        std::fprintf(f,
+                     "{\"closure\": %d, \"n_states\": %d, \"n_moments\": %d, \"n_quad\": %d, \"precision\": %d, "
+                     "\"waves\": %d, \"blocks_per_cu\": %d, \"ms\": %.6f, \"cells\": %lld}\n",
+                     ctx->kind, ctx->S, ctx->M, ctx->Q, ctx->precision, best.waves, best.blocks_per_cu, best_ms,
+                     (long long)n);
+        std::fclose(f);
+    }
+    return timed;
+}
+
+int uqk_load_tuning(uqk_ctx* ctx, const char* json_path) {
+    if (!ctx || !json_path) return -1;
+    FILE* f = std::fopen(json_path, "r");
+    if (!f) return fail(ctx, -1, std::string("uqk_load_tuning: cannot open ") + json_path);
+    char line[512];
+    int found = 0;
+    while (std::fgets(line, sizeof(line), f)) {
+        int k = -1, S = -1, M = -1, Q = -1, p = -1, w = 0, b = 0;
+        if (std::sscanf(line,
+                        "{\"closure\": %d, \"n_states\": %d, \"n_moments\": %d, \"n_quad\": %d, \"precision\": %d, "
+                        "\"waves\": %d, \"blocks_per_cu\": %d",
+                        &k, &S, &M, &Q, &p, &w, &b) == 7 &&
+            k == ctx->kind && S == ctx->S && M == ctx->M && Q == ctx->Q && p == ctx->precision) {
+            ctx->shape.waves = w;  // the last matching line wins
+            ctx->shape.blocks_per_cu = b;
+            found = 1;
+        }
+    }
+    std::fclose(f);
+    return found;
 }
 
 }  // extern "C"

@@ -40,6 +40,7 @@ struct uqk_ctx {
     std::string graph_key;
     size_t max_lds = 0;
     int num_cu = 0;
+    uqk::LaunchShape shape;   // specialised closure kernel launch shape (autotuned or set)
     uqk_mlp* warm = nullptr;  // learned warm start run before each closure (not owned)
     std::string err;
 };

@@ -40,8 +40,19 @@ struct RunState {
     int active;
 };
 
+// Launch shape of the specialised closure kernel; 0 = automatic (as many
+// waves per workgroup as LDS allows, every resident workgroup slot filled).
+struct LaunchShape {
+    int waves = 0;          // wavefronts (cells in flight) per workgroup: 1, 2 or 4
+    int blocks_per_cu = 0;  // persistent workgroups per CU (capped by occupancy)
+};
+
 bool newton_wave_launch(int kind, int M, int Q, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
-                        hipError_t* err);
+                        hipError_t* err, const LaunchShape& shape = LaunchShape());
+// Largest waves per workgroup the LDS allows and the resident workgroups per
+// CU at `waves` for the specialised kernel of (kind, M, Q, prec); false when
+// there is none.
+bool newton_wave_limits(int kind, int M, int Q, int prec, int max_lds, int waves, int* max_waves, int* occupancy);
 int newton_fp64_wave_doubles(int S, int M, int Q);
 int newton_fp64_occupancy(int kind, bool tlds, int threads, size_t shmem);
 hipError_t newton_fp64_launch(int kind, const NewtonArgs& a, bool tlds, int blocks, int threads, size_t shmem,

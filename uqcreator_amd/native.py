@@ -100,6 +100,10 @@ SIGNATURES = {
     "uqk_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32]),
     "uqk_device_synchronize": (C.c_int, []),
     "uqk_device_count": (C.c_int32, []),
+    "uqk_set_launch_shape": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "uqk_get_launch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "uqk_autotune": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_char_p]),
+    "uqk_load_tuning": (C.c_int, [C.c_void_p, C.c_char_p]),
     "uqk_mlp_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
     "uqk_mlp_destroy": (C.c_int, [C.c_void_p]),
     "uqk_mlp_shape": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]),
@@ -299,6 +303,23 @@ class GpuClosure:
 
     def error(self):
         return lib().uqk_last_error(self.h).decode()
+
+    def set_launch_shape(self, waves=0, blocks_per_cu=0):
+        self._check(lib().uqk_set_launch_shape(self.h, waves, blocks_per_cu), "uqk_set_launch_shape")
+
+    def launch_shape(self):
+        w, b = C.c_int32(), C.c_int32()
+        lib().uqk_get_launch_shape(self.h, C.byref(w), C.byref(b))
+        return w.value, b.value
+
+    def autotune(self, u, lam, json_path=None):
+        """Times every launch shape of the specialised kernel on these cells
+        and keeps the fastest; returns the number of shapes timed."""
+        return self._check(lib().uqk_autotune(self.h, u.shape[0], _p(u), _p(lam),
+                                              json_path.encode() if json_path else None), "uqk_autotune")
+
+    def load_tuning(self, json_path):
+        return self._check(lib().uqk_load_tuning(self.h, json_path.encode()), "uqk_load_tuning")
 
     def set_kernel(self, kernel):
         """KERNEL_AUTO, KERNEL_GENERIC or KERNEL_WAVE."""
