@@ -57,6 +57,20 @@ def test_euler1d_unsafe_and_one_shot(oracle, kernel):
         c = oracle.closure(oracle.EULER1D, **{"du_exact": 1, **okw})
         lam_o = lam0.copy()
         st_o, it_o = oracle.solve_closure(c, b.phi, b.phiw, u, lam_o)
+        if kw.get("du_exact") == 0:
+            # the shipped DU(2,2) leaves most Hessians indefinite by a margin
+            # near rounding; the blocked factorisation (n > 32) sums the
+            # trailing updates in another order than the unblocked one, so the
+            # not-SPD verdict may flip in a borderline cell.  Every cell the
+            # GPU reports converged must be a solution.
+            assert (st != st_o).mean() <= 0.01, np.flatnonzero(st != st_o)
+            for j in np.flatnonzero((st == 0) & (st_o != 0)):
+                assert np.linalg.norm(oracle.gradient(c, b.phi, b.phiw, lam[j], u[j])) < 1e-10
+            ok = (st == 0) & (st_o == 0)
+            assert nfail == (st != 0).sum()
+            scale = np.abs(lam_star).max(axis=1, keepdims=True)
+            assert (np.abs(lam - lam_o)[ok] / scale[ok]).max() < 1e-8
+            continue
         np.testing.assert_array_equal(st, st_o)
         ok = st == 0
         assert nfail == (~ok).sum()

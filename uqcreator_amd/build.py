@@ -22,6 +22,9 @@ HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions stay off (they have given silent wrong results)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in VGPRs (not AGPRs): the accumulator registers are
+    # reused by the other phases, which keeps the closure kernels at 2+ waves/SIMD
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-munsafe-fp-atomics", "-Wall", "-Wno-unused-result",
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
 ]

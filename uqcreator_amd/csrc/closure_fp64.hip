@@ -67,11 +67,31 @@ __global__ void __launch_bounds__(256) newton_fp64_kernel(NewtonArgs a) {
 
     const ClosureParams& c = a.c;
 
+    // Work queue, guided: a wave takes (remaining / 2 waves) queue positions
+    // with one atomicAdd, so chunks start large (few atomics) and shrink to
+    // single positions at the end (balance).  Position t of the 2^qbits queue
+    // is cell bitreverse(t) (skipped when >= ncell): neighbouring costly cells
+    // go to different chunks.  Newton and failure counts are summed per wave
+    // (wave-uniform, scalar) and added once at the end.
+    unsigned long long w_fail = 0, w_newton = 0, w_grad = 0;
+    const long qsize = 1L << a.qbits;
+    long qpos = 0, qend = 0;
     for (;;) {
-        long cell = 0;
-        if (lane == 0) cell = (long)atomicAdd(a.counter, 1ull);
-        cell = __shfl(cell, 0, 64);
-        if (cell >= a.ncell) break;
+        if (qpos == qend) {
+            long want = (qsize - qend) / (2 * a.waves);
+            if (want < 1) want = 1;
+            unsigned long long got = 0;
+            if (lane == 0) got = atomicAdd(a.counter, (unsigned long long)want);
+            const long base = (long)(((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(got >> 32)) << 32) |
+                                     (unsigned)__builtin_amdgcn_readfirstlane((unsigned)got));
+            if (base >= qsize) break;
+            qpos = base;
+            qend = base + want < qsize ? base + want : qsize;
+        }
+        const unsigned pos = (unsigned)qpos;
+        ++qpos;
+        const long cell = a.qbits ? (long)(__builtin_bitreverse32(pos) >> (32 - a.qbits)) : 0;
+        if (cell >= a.ncell) continue;
         const int nt = a.nt_cell ? a.nt_cell[cell] : M;
         const int n = S * nt;
         const double* ug = a.u + cell * SM;
@@ -262,11 +282,17 @@ __global__ void __launch_bounds__(256) newton_fp64_kernel(NewtonArgs a) {
         if (lane == 0) {
             if (a.status) a.status[cell] = status;
             if (a.iters) a.iters[cell] = iters;
-            if (status != UQK_STATUS_CONVERGED) atomicAdd(a.nfail, 1ull);
-            atomicAdd(a.nfail + 1, (unsigned long long)iters);
-            atomicAdd(a.nfail + 2, (unsigned long long)ngrad);
         }
+        // status, iters and ngrad are wave-uniform: these sums stay scalar
+        w_fail += status != UQK_STATUS_CONVERGED ? 1ull : 0ull;
+        w_newton += (unsigned long long)iters;
+        w_grad += (unsigned long long)ngrad;
         wave_sync();
+    }
+    if (lane == 0) {
+        if (w_fail) atomicAdd(a.nfail, w_fail);
+        if (w_newton) atomicAdd(a.nfail + 1, w_newton);
+        if (w_grad) atomicAdd(a.nfail + 2, w_grad);
     }
 }
 
@@ -309,8 +335,10 @@ int newton_fp64_occupancy(int kind, bool tlds, int threads, size_t shmem) {
     return 0;
 }
 
-hipError_t newton_fp64_launch(int kind, const NewtonArgs& a, bool tlds, int blocks, int threads, size_t shmem,
+hipError_t newton_fp64_launch(int kind, const NewtonArgs& args, bool tlds, int blocks, int threads, size_t shmem,
                               hipStream_t st) {
+    NewtonArgs a = args;
+    set_queue(a, (long)blocks * (threads / 64));
     switch (kind) {
         case UQK_BOUNDED_BARRIER: return launch_kind<UQK_BOUNDED_BARRIER>(a, tlds, blocks, threads, shmem, st);
         case UQK_EULER1D: return launch_kind<UQK_EULER1D>(a, tlds, blocks, threads, shmem, st);

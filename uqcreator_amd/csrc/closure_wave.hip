@@ -30,19 +30,8 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// offset of H(r, c), r <= c: packed upper triangle, or element (c, r) of the
-// lower 16 x 16 tile (c/16, r/16) in the blocked layout
-template <bool BLK>
-__device__ __forceinline__ int hidx(int r, int c) {
-    if constexpr (BLK) {
-        const int rb = c >> 4, cb = r >> 4;
-        return (rb * (rb + 1) / 2 + cb) * 256 + (c & 15) * 16 + (r & 15);
-    } else {
-        return c * (c + 1) / 2 + r;
-    }
-}
-
-__device__ __forceinline__ int tile(int rb, int cb) { return (rb * (rb + 1) / 2 + cb) * 256; }
+// offset of H(r, c), r <= c, in the packed upper triangle
+__device__ __forceinline__ int hidx(int r, int c) { return c * (c + 1) / 2 + r; }
 
 // 1/sqrt(h) from v_rsq_f64 refined by two Newton steps
 __device__ __forceinline__ double rsqrt_f64(double h) {
@@ -52,7 +41,6 @@ __device__ __forceinline__ double rsqrt_f64(double h) {
     return r;
 }
 
-typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 template <int KIND, int M, int Q>
 struct WaveShape {
@@ -62,20 +50,16 @@ struct WaveShape {
     static constexpr int QP = (Q + 1) & ~1;   // padded row length (b128 reads)
     static constexpr int NPAIR = M * (M + 1) / 2;
     static constexpr int HP = N * (N + 1) / 2; // packed upper triangle
-    // n > 32: blocked Cholesky on 16 x 16 tiles (lower tiles stored row-major)
-    static constexpr bool BLK = N > 32;
-    static constexpr int NBLK = (N + 15) / 16;
-    static constexpr int NP = 16 * NBLK;
-    static constexpr int HSZ = BLK ? NBLK * (NBLK + 1) / 2 * 256 : HP;
     // per-wave LDS doubles
     static constexpr int LAM = 0;
     static constexpr int LAMNEW = LAM + S * M;
     static constexpr int UU = LAMNEW + S * M;
     static constexpr int UB = UU + S * M;        // U values, S rows of QP
     static constexpr int HS = UB + S * QP;       // packed H / U factor
-    static constexpr int YB = HS + HSZ;          // 64
+    static constexpr int YB = HS + HP;           // 64
     static constexpr int DIAG = YB + 64;         // 64 reciprocal pivots
-    static constexpr int WAVE_DOUBLES = DIAG + 64;
+    static constexpr int ROW = DIAG + 64;        // 2 x 64: row j of U, double buffered
+    static constexpr int WAVE_DOUBLES = ROW + 128;
     static constexpr int TABLE_DOUBLES = 2 * M * QP;  // phiT, phiwT (i-major)
 };
 
@@ -109,6 +93,7 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
     double* Hs = wb + W::HS;
     double* yb = wb + W::YB;
     double* rdiag = wb + W::DIAG;
+    double* rowb = wb + W::ROW;
 
     // lane k evaluates quadrature point k (lanes past Q repeat the last point)
     const int kq = lane < Q ? lane : Q - 1;
@@ -130,13 +115,6 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
     }
     const ClosureParams& c = a.c;
     const int SM = S * M;
-    if constexpr (W::BLK) {
-        // identity on the padding rows n..NP-1; the Hessian never writes them and
-        // the factorisation leaves them unchanged
-        for (int i = N; i < W::NP; ++i)
-            for (int cc = lane; cc <= i; cc += 64) Hs[hidx<true>(cc, i)] = (cc == i) ? 1.0 : 0.0;
-        wave_sync();
-    }
 
     // MFMA operands of the Hessian blocks H_lm = Phi^T diag(DU_lm) PhiW:
     // A(j, k) = phi(k, j) and the constant factor PhiW(k, i) of B(k, i), in the
@@ -156,11 +134,31 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
         __syncthreads();
     }
 
+    // Work queue, guided: a wave takes (remaining / 2 waves) queue positions
+    // with one atomicAdd, so chunks start large (few atomics) and shrink to
+    // single positions at the end (balance).  Position t of the 2^qbits queue
+    // is cell bitreverse(t) (skipped when >= ncell): neighbouring costly cells
+    // go to different chunks.  Newton and failure counts are summed per wave
+    // (wave-uniform, scalar) and added once at the end.
+    unsigned long long w_fail = 0, w_newton = 0, w_grad = 0;
+    const long qsize = 1L << a.qbits;
+    long qpos = 0, qend = 0;
     for (;;) {
-        long cell = 0;
-        if (lane == 0) cell = (long)atomicAdd(a.counter, 1ull);
-        cell = __shfl(cell, 0, 64);
-        if (cell >= a.ncell) break;
+        if (qpos == qend) {
+            long want = (qsize - qend) / (2 * a.waves);
+            if (want < 1) want = 1;
+            unsigned long long got = 0;
+            if (lane == 0) got = atomicAdd(a.counter, (unsigned long long)want);
+            const long base = (long)(((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(got >> 32)) << 32) |
+                                     (unsigned)__builtin_amdgcn_readfirstlane((unsigned)got));
+            if (base >= qsize) break;
+            qpos = base;
+            qend = base + want < qsize ? base + want : qsize;
+        }
+        const unsigned pos = (unsigned)qpos;
+        ++qpos;
+        const long cell = a.qbits ? (long)(__builtin_bitreverse32(pos) >> (32 - a.qbits)) : 0;
+        if (cell >= a.ncell) continue;
         const double* ug = a.u + cell * SM;
         double* lg = a.lam + cell * SM;
         for (int t = lane; t < SM; t += 64) {
@@ -257,7 +255,7 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
                                 const int j = 16 * tj + 4 * (lane >> 4) + e, i = 16 * ti + (lane & 15);
                                 if (j < M && i < M && (m < l || j <= i)) {
                                     const int r = m * M + j, cc = l * M + i;
-                                    Hs[hidx<W::BLK>(r, cc)] = (double)acc[l * (l + 1) / 2 + m][tj][ti][e];
+                                    Hs[hidx(r, cc)] = (double)acc[l * (l + 1) / 2 + m][tj][ti][e];
                                 }
                             }
             wave_sync();
@@ -307,10 +305,10 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
                         for (int m = 0; m <= l; ++m) {
                             double v = acc[q][l * (l + 1) / 2 + m];
                             int r1 = m * M + j, c1 = l * M + i;
-                            Hs[hidx<W::BLK>(r1, c1)] = v;
+                            Hs[hidx(r1, c1)] = v;
                             if (m < l) {
                                 int r2 = m * M + i, c2 = l * M + j;
-                                Hs[hidx<W::BLK>(r2, c2)] = v;
+                                Hs[hidx(r2, c2)] = v;
                             }
                         }
                 }
@@ -319,120 +317,14 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
             // phase end: hessian
         };
 
-        // d <- H^{-1} g with H = L L^T factored in 16-column panels: each panel
-        // is eliminated with lane-per-row registers and SGPR broadcasts (the
-        // right-hand side rides along as the forward substitution); the trailing
-        // tiles are updated with FP64 MFMA (v_mfma_f64_16x16x4f64), operands
-        // straight from the LDS tiles.  Returns false if H is not positive definite.
-        auto cholesky_blocked = [&](double g, double& d) -> bool {
-            // phase: factor
-            constexpr int NB = W::NBLK, NP = W::NP;
-            double* gv = yb;     // right-hand side, then y, then x
-            double* rd = rdiag;  // reciprocal pivots
-            gv[lane] = lane < N ? g : 0.0;
-            wave_sync();
-            const int lr = lane & 15, lk = lane >> 4;
-#pragma unroll
-            for (int kb = 0; kb < NB; ++kb) {
-                const int R = NP - 16 * kb;  // rows of this panel
-                const int gr = 16 * kb + lane;
-                double Pn[16];
-                double G = 0.0;
-                if (lane < R) {
-                    const double* row = Hs + tile(gr >> 4, kb) + (gr & 15) * 16;
-#pragma unroll
-                    for (int cc = 0; cc < 16; ++cc) Pn[cc] = row[cc];
-                    G = gv[gr];
-                } else {
-#pragma unroll
-                    for (int cc = 0; cc < 16; ++cc) Pn[cc] = 0.0;
-                }
-                bool ok = true;
-#pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const double hjj = readlane_f64(Pn[j], j);
-                    if (!(hjj > 0.0)) {
-                        ok = false;
-                        break;
-                    }
-                    const double rinv = rsqrt_f64(hjj);
-                    if (lane == 0) rd[16 * kb + j] = rinv;
-                    const bool below = lane > j && lane < R;
-                    Pn[j] = (lane == j) ? hjj * rinv : (below ? Pn[j] * rinv : Pn[j]);
-                    const double yj = readlane_f64(G, j) * rinv;
-                    G = (lane == j) ? yj : (below ? G - Pn[j] * yj : G);
-                    const double lj = Pn[j];
-#pragma unroll
-                    for (int cc = j + 1; cc < 16; ++cc) {
-                        const double lcj = readlane_f64(lj, cc);
-                        if (below) Pn[cc] -= lj * lcj;
-                    }
-                }
-                if (!ok) return false;
-                if (lane < R) {
-                    double* row = Hs + tile(gr >> 4, kb) + (gr & 15) * 16;
-#pragma unroll
-                    for (int cc = 0; cc < 16; ++cc) row[cc] = Pn[cc];
-                    gv[gr] = G;
-                }
-                wave_sync();
-                // trailing tiles: C(rb, cb) -= L(rb, kb) L(cb, kb)^T
-#pragma unroll
-                for (int rb = kb + 1; rb < NB; ++rb)
-#pragma unroll
-                    for (int cb = kb + 1; cb <= rb; ++cb) {
-                        double* C = Hs + tile(rb, cb);
-                        const double* A = Hs + tile(rb, kb);
-                        const double* B = Hs + tile(cb, kb);
-                        f64x4 acc;
-#pragma unroll
-                        for (int e = 0; e < 4; ++e) acc[e] = C[(4 * e + lk) * 16 + lr];
-#pragma unroll
-                        for (int ks = 0; ks < 4; ++ks)
-                            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-A[lr * 16 + 4 * ks + lk], B[lr * 16 + 4 * ks + lk],
-                                                                      acc, 0, 0, 0);
-#pragma unroll
-                        for (int e = 0; e < 4; ++e) C[(4 * e + lk) * 16 + lr] = acc[e];
-                    }
-                wave_sync();
-            }
-            // phase end: factor
-            // phase: backsub
-            // L^T x = y, tile column by tile column from the last
-#pragma unroll
-            for (int kb = NB - 1; kb >= 0; --kb) {
-                double t = 0.0;
-                if (lane < 16) {
-                    t = gv[16 * kb + lane];
-#pragma unroll
-                    for (int rb = kb + 1; rb < NB; ++rb) {
-                        const double* A = Hs + tile(rb, kb);
-#pragma unroll
-                        for (int r = 0; r < 16; ++r) t -= A[r * 16 + lane] * gv[16 * rb + r];
-                    }
-                }
-                const double* D = Hs + tile(kb, kb);
-#pragma unroll
-                for (int i = 15; i >= 0; --i) {
-                    const double xi = readlane_f64(t, i) * rd[16 * kb + i];
-                    if (lane < i) t -= D[i * 16 + lane] * xi;
-                    if (lane == i) t = xi;
-                }
-                wave_sync();
-                if (lane < 16) gv[16 * kb + lane] = t;
-                wave_sync();
-            }
-            d = lane < N ? gv[lane] : 0.0;
-            wave_sync();
-            // phase end: backsub
-            return true;
-        };
-
         // d <- H^{-1} g; returns false if H is not positive definite
         auto cholesky_solve = [&](double g, double& d) -> bool {
-            if constexpr (W::BLK) return cholesky_blocked(g, d);
             // phase: factor
-            // lane c < N: column c of H; lane N: right-hand side
+            // lane c < N: column c of H; lane N: right-hand side.  Step j
+            // scales row j of U (lane r holds U(j, r)) and every lane reads it
+            // back from LDS as broadcasts (ds_read2_b64) for its rank-1 update;
+            // the row buffer alternates so a step's reads never race the next
+            // step's writes.
             double Hc[N];
             yb[lane] = g;
             wave_sync();
@@ -452,14 +344,15 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
                     ok = false;
                     break;
                 }
-                const double ujj = sqrt(hjj);
-                const double rinv = 1.0 / ujj;
+                const double rinv = rsqrt_f64(hjj);
                 if (lane == 0) rdiag[j] = rinv;
-                Hc[j] = (lane == j) ? ujj : Hc[j] * rinv;
+                Hc[j] = (lane == j) ? hjj * rinv : Hc[j] * rinv;
                 const double hj = Hc[j];
-                // row j of U, U(j, r) held by lane r, broadcast through SGPRs
+                double* urow = rowb + 64 * (j & 1);
+                urow[lane] = hj;
+                wave_sync();
 #pragma unroll
-                for (int r = j + 1; r < N; ++r) Hc[r] -= readlane_f64(hj, r) * hj;
+                for (int r = j + 1; r < N; ++r) Hc[r] -= urow[r] * hj;
             }
             if (!ok) return false;
             // phase end: factor
@@ -557,11 +450,17 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
         if (lane == 0) {
             if (a.status) a.status[cell] = status;
             if (a.iters) a.iters[cell] = iters;
-            if (status != UQK_STATUS_CONVERGED) atomicAdd(a.nfail, 1ull);
-            atomicAdd(a.nfail + 1, (unsigned long long)iters);
-            atomicAdd(a.nfail + 2, (unsigned long long)ngrad);
         }
+        // status, iters and ngrad are wave-uniform: these sums stay scalar
+        w_fail += status != UQK_STATUS_CONVERGED ? 1ull : 0ull;
+        w_newton += (unsigned long long)iters;
+        w_grad += (unsigned long long)ngrad;
         wave_sync();
+    }
+    if (lane == 0) {
+        if (w_fail) atomicAdd(a.nfail, w_fail);
+        if (w_newton) atomicAdd(a.nfail + 1, w_newton);
+        if (w_grad) atomicAdd(a.nfail + 2, w_grad);
     }
 }
 
@@ -606,7 +505,9 @@ static void wave_go(const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st
     long want = (a.ncell + waves - 1) / waves;
     int blocks = (int)(want < (long)per_cu * num_cu ? want : (long)per_cu * num_cu);
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL((newton_wave_kernel<KIND, M, Q, PREC>), dim3(blocks), dim3(threads), shmem, st, a);
+    NewtonArgs b = a;
+    set_queue(b, (long)blocks * waves);
+    hipLaunchKernelGGL((newton_wave_kernel<KIND, M, Q, PREC>), dim3(blocks), dim3(threads), shmem, st, b);
     *err = hipGetLastError();
 }
 

@@ -11,10 +11,11 @@
 // Features and targets (Euler closures, S = 3 or 4 states): from the cell
 // means rho0, m0, E0 take p0 = (gamma-1)(E0 - |m0|^2/(2 rho0)), T0 = p0/rho0
 // and the scales D = (rho0, rho0 sqrt(T0) [per momentum], rho0 T0).  The
-// input is u(i,s)/D_s; the target is (lambda - lambda_DS)(i,s) * D_s/rho0
-// with lambda_DS = (DS(u_0), 0, ...) the solver's plain start.  Scaling u by
-// D maps the dual of the scaled problem affinely onto the original one, so
-// the inverse map lambda = lambda_DS + y * rho0/D_s is exact for any scale.
+// input is u(i,s)/D_s; the target is (lambda - lambda_lin)(i,s) * D_s/rho0
+// with lambda_lin the linearised start (LinearStart below: the plain start
+// (DS(u_0), 0, ...) followed by one Newton step, in closed form).  Scaling u
+// by D maps the dual of the scaled problem affinely onto the original one,
+// so the inverse map lambda = lambda_lin + y * rho0/D_s holds at any scale.
 // Both are standardised per feature with the training set's mean and std.
 #include <hip/hip_runtime.h>
 
@@ -236,13 +237,56 @@ __device__ __forceinline__ bool euler_scales(const double* u, double gm, double*
     return true;
 }
 
+// Linearised start: at lambda = (DS(u_0), 0, ...) every quadrature point
+// sees the same Lambda, so the gradient is (0, -u_1, ..., -u_{M-1}) and the
+// Hessian is I_M (x) DU(DS(u_0)) (orthonormal basis).  One full Newton step
+// from the plain start is therefore lambda_i = DU^{-1} u_i (i >= 1) with
+// lambda_0 = DS(u_0) unchanged -- closed form, S x S per cell.  The network
+// learns what remains.  LinearStart factors DU once; solve() gives z_i.
+template <int KIND>
+struct LinearStart {
+    static constexpr int S = uqk::KindInfo<KIND>::S;
+    double L[S][S];
+    __device__ bool factor(const uqk::ClosureParams& cp, const double* ds) {
+        double y[S * (S + 1) / 2];
+        uqk::closure_DU<KIND>(cp, ds, y);
+        for (int l = 0; l < S; ++l)
+            for (int m = 0; m <= l; ++m) L[l][m] = y[l * (l + 1) / 2 + m];
+        for (int j = 0; j < S; ++j) {
+            double d = L[j][j];
+            for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
+            if (!(d > 0.0) || !isfinite(d)) return false;
+            L[j][j] = sqrt(d);
+            for (int i = j + 1; i < S; ++i) {
+                double v = L[i][j];
+                for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
+                L[i][j] = v / L[j][j];
+            }
+        }
+        return true;
+    }
+    __device__ void solve(const double* b, double* z) const {
+        double w[S];
+        for (int i = 0; i < S; ++i) {
+            double v = b[i];
+            for (int k = 0; k < i; ++k) v -= L[i][k] * w[k];
+            w[i] = v / L[i][i];
+        }
+        for (int i = S - 1; i >= 0; --i) {
+            double v = w[i];
+            for (int k = i + 1; k < S; ++k) v -= L[k][i] * z[k];
+            z[i] = v / L[i][i];
+        }
+    }
+};
+
 struct FeatArgs {
     long n;                   // rows
     const long* idx;          // row -> cell (null: identity)
     const double* u;          // cells x M x S
     const double* lam;        // cells x M x S (null: no targets)
     int M;
-    double gamma;
+    uqk::ClosureParams cp;
     int K0, NO;
     const float* mu_x;        // [K0] (null: raw features)
     const float* sd_x;
@@ -253,14 +297,16 @@ struct FeatArgs {
     float* tf;                // rows x NO, targets (raw if mu_t null)
 };
 
-template <int S>
+template <int KIND>
 __global__ void features_kernel(FeatArgs a) {
+    constexpr int S = uqk::KindInfo<KIND>::S;
     const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= a.n) return;
     const long c = a.idx ? a.idx[r] : r;
     const double* u = a.u + (size_t)c * a.M * S;
     double D[S], ds[S];
-    const bool ok = euler_scales<S>(u, a.gamma, D, ds);
+    LinearStart<KIND> lin;
+    const bool ok = euler_scales<S>(u, a.cp.gamma, D, ds) && lin.factor(a.cp, ds);
     const int nf = a.M * S;
     for (int f = 0; f < a.K0; ++f) {
         float x = 0.f;
@@ -271,39 +317,57 @@ __global__ void features_kernel(FeatArgs a) {
         if (a.xh) a.xh[(size_t)r * a.K0 + f] = (__bf16)x;
         if (a.xf) a.xf[(size_t)r * a.K0 + f] = x;
     }
-    if (a.lam) {
-        const double* l = a.lam + (size_t)c * a.M * S;
-        for (int f = 0; f < a.NO; ++f) {
+    if (!a.lam) return;
+    const double* l = a.lam + (size_t)c * a.M * S;
+    float* tr = a.tf + (size_t)r * a.NO;
+    for (int i = 0; i < a.M; ++i) {
+        double z[S];
+        if (i == 0) {
+            for (int s = 0; s < S; ++s) z[s] = ds[s];
+        } else if (ok) {
+            lin.solve(u + i * S, z);
+        }
+        for (int s = 0; s < S; ++s) {
+            const int f = i * S + s;
             float t = 0.f;
-            if (ok && f < nf) {
-                const int s = f % S;
-                const double base = f < S ? ds[s] : 0.0;
-                t = (float)((l[f] - base) * D[s] / D[0]);
+            if (ok) {
+                t = (float)((l[f] - z[s]) * D[s] / D[0]);
                 if (a.mu_t) t = (t - a.mu_t[f]) / a.sd_t[f];
             }
-            a.tf[(size_t)r * a.NO + f] = t;
+            tr[f] = t;
         }
     }
+    for (int f = nf; f < a.NO; ++f) tr[f] = 0.f;
 }
 
-// lambda = lambda_DS + (mu_t + sd_t y) rho0/D_s; NaN where the mean state is
+// lambda = lambda_lin + (mu_t + sd_t y) rho0/D_s; NaN where the mean state is
 // not physical (the judge then keeps the plain start).
-template <int S>
-__global__ void apply_kernel(long n, const double* u, const float* y, int M, double gamma, int NO, const float* mu_t,
-                             const float* sd_t, double* lam) {
+template <int KIND>
+__global__ void apply_kernel(long n, const double* u, const float* y, int M, uqk::ClosureParams cp, int NO,
+                             const float* mu_t, const float* sd_t, double* lam) {
+    constexpr int S = uqk::KindInfo<KIND>::S;
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const double* uc = u + (size_t)c * M * S;
     double D[S], ds[S];
-    const bool ok = euler_scales<S>(uc, gamma, D, ds);
-    for (int f = 0; f < M * S; ++f) {
-        const int s = f % S;
-        double v = NAN;
-        if (ok) {
-            const double t = (double)mu_t[f] + (double)sd_t[f] * (double)y[(size_t)c * NO + f];
-            v = (f < S ? ds[s] : 0.0) + t * D[0] / D[s];
+    LinearStart<KIND> lin;
+    const bool ok = euler_scales<S>(uc, cp.gamma, D, ds) && lin.factor(cp, ds);
+    for (int i = 0; i < M; ++i) {
+        double z[S];
+        if (i == 0) {
+            for (int s = 0; s < S; ++s) z[s] = ds[s];
+        } else if (ok) {
+            lin.solve(uc + i * S, z);
         }
-        lam[(size_t)c * M * S + f] = v;
+        for (int s = 0; s < S; ++s) {
+            const int f = i * S + s;
+            double v = NAN;
+            if (ok) {
+                const double t = (double)mu_t[f] + (double)sd_t[f] * (double)y[(size_t)c * NO + f];
+                v = z[s] + t * D[0] / D[s];
+            }
+            lam[(size_t)c * M * S + f] = v;
+        }
     }
 }
 
@@ -568,7 +632,7 @@ int launch_features(uqk_mlp* n, long rows, const long* idx, const double* u, con
     a.u = u;
     a.lam = lam;
     a.M = n->M;
-    a.gamma = n->ctx->c.gamma;
+    a.cp = n->ctx->c;
     a.K0 = n->K0;
     a.NO = n->NO;
     if (standardise) {
@@ -580,10 +644,10 @@ int launch_features(uqk_mlp* n, long rows, const long* idx, const double* u, con
     a.xh = xh;
     a.xf = xf;
     a.tf = tf;
-    if (n->S == 3)
-        hipLaunchKernelGGL(features_kernel<3>, dim3(blocks(rows)), dim3(256), 0, n->ctx->stream, a);
+    if (n->ctx->kind == UQK_EULER1D)
+        hipLaunchKernelGGL(features_kernel<UQK_EULER1D>, dim3(blocks(rows)), dim3(256), 0, n->ctx->stream, a);
     else
-        hipLaunchKernelGGL(features_kernel<4>, dim3(blocks(rows)), dim3(256), 0, n->ctx->stream, a);
+        hipLaunchKernelGGL(features_kernel<UQK_EULER2D>, dim3(blocks(rows)), dim3(256), 0, n->ctx->stream, a);
     MLP_HIP(n, hipGetLastError());
     return 0;
 }
@@ -606,12 +670,12 @@ int enqueue_predict(uqk_mlp* n, long ncell, const double* u, double* lambda_out)
         if (int rc = launch_features(n, rows, nullptr, uc, nullptr, true, n->act[0], nullptr, nullptr)) return rc;
         if (int rc = forward(n, rows, false)) return rc;
         double* lc = lambda_out + (size_t)c0 * n->M * n->S;
-        if (n->S == 3)
-            hipLaunchKernelGGL(apply_kernel<3>, dim3(blocks(rows)), dim3(256), 0, st, rows, uc, n->y, n->M,
-                               n->ctx->c.gamma, n->NO, mu_t, sd_t, lc);
+        if (n->ctx->kind == UQK_EULER1D)
+            hipLaunchKernelGGL(apply_kernel<UQK_EULER1D>, dim3(blocks(rows)), dim3(256), 0, st, rows, uc, n->y,
+                               n->M, n->ctx->c, n->NO, mu_t, sd_t, lc);
         else
-            hipLaunchKernelGGL(apply_kernel<4>, dim3(blocks(rows)), dim3(256), 0, st, rows, uc, n->y, n->M,
-                               n->ctx->c.gamma, n->NO, mu_t, sd_t, lc);
+            hipLaunchKernelGGL(apply_kernel<UQK_EULER2D>, dim3(blocks(rows)), dim3(256), 0, st, rows, uc, n->y,
+                               n->M, n->ctx->c, n->NO, mu_t, sd_t, lc);
     }
     MLP_HIP(n, hipGetLastError());
     return 0;

@@ -27,6 +27,8 @@ struct NewtonArgs {
     int wave_doubles;            // LDS doubles per wave
     int prec;                    // UQK_FP64 or UQK_MIXED (Hessian arithmetic)
     const int* active;           // device flag: 0 skips the launch (graph replays past t_end)
+    long waves;                  // resident waves of the launch (work-queue chunking), set by the launcher
+    int qbits;                   // work queue of 2^qbits positions >= ncell, set by the launcher
 };
 
 // Device-side state of a multi-step run replayed from a captured graph.
@@ -46,6 +48,14 @@ struct LaunchShape {
     int waves = 0;          // wavefronts (cells in flight) per workgroup: 1, 2 or 4
     int blocks_per_cu = 0;  // persistent workgroups per CU (capped by occupancy)
 };
+
+// Work-queue parameters of a closure launch with `resident_waves` waves.
+inline void set_queue(NewtonArgs& a, long resident_waves) {
+    a.waves = resident_waves > 0 ? resident_waves : 1;
+    int bits = 0;
+    while ((1L << bits) < a.ncell) ++bits;
+    a.qbits = bits;
+}
 
 bool newton_wave_launch(int kind, int M, int Q, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
                         hipError_t* err, const LaunchShape& shape = LaunchShape());

@@ -115,3 +115,41 @@ def sw2d_c5(n=1024, degree=4, q1=5):
     m = _mesh.quad_grid(n, n, -1.0, 1.0, -1.0, 1.0, native.BC_DIRICHLET)
     u = project_ic(sw2d_dambreak_ic(), m.center, b)
     return b, m, u
+
+
+def cell_orders(n_cells, seed=SEED, n_max=6):
+    """Per-cell moment order N in {1..n_max}, uniform, from numpy's Philox
+    generator keyed by the seed: the uneven per-cell Newton sizes of an
+    adaptive run, reproducible from the seed."""
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    return 1 + rng.integers(0, n_max, size=n_cells)
+
+
+def euler2d_c4(ni=512, nj=128, seed=SEED):
+    """Config: the 2D Euler NACA0012 setup with per-cell orders N in 1..6
+    (graded total-degree levels, n_total = 3, 6, 10, 15, 21, 28), 7x7 Gauss.
+    Returns (basis, mesh, u, lam0, nt_cell)."""
+    from . import mesh as _mesh
+
+    b = native.Basis(2, [1, 2, 3, 4, 5, 6], 7)
+    m = _mesh.naca_ogrid(ni, nj)
+    u = project_ic(euler2d_freestream_ic(), m.center, b)
+    orders = cell_orders(m.n_cells, seed)
+    nt = np.asarray(b.ntotal_per_level, dtype=np.int32)[orders - 1]
+    lam0 = np.zeros_like(u)
+    lam0[:, 0] = euler2d_ds(u[:, 0])
+    return b, m, u, lam0, nt
+
+
+def euler2d_ds(u0, gamma=1.4):
+    """EulerClosure2D::DS (eulerclosure2d.cpp:90-102) of the mean states
+    u0 (n, 4): the plain Newton start lambda_0 = (DS(u_0), 0, ...)."""
+    g = -gamma
+    rho, ru, rv, rE = u0[:, 0], u0[:, 1], u0[:, 2], u0[:, 3]
+    q2 = ru * ru + rv * rv
+    out = np.empty_like(u0)
+    out[:, 0] = (q2 + g * (2 * rho * rE - q2)) / (-2 * rho * rE + q2) - np.log(rho ** g * (rE - q2 / (2 * rho)))
+    out[:, 1] = -((2 * rho * ru) / (-2 * rho * rE + q2))
+    out[:, 2] = -((2 * rho * rv) / (-2 * rho * rE + q2))
+    out[:, 3] = -(rho / (rE - q2 / (2 * rho)))
+    return out
