@@ -69,7 +69,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // (v_mfma_f32_16x16x4f32, FP32 operands and accumulation); gradient, residual,
 // accept test, Cholesky and solve stay FP64.
 template <int KIND, int M, int Q, int PREC>
-__global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) newton_wave_kernel(NewtonArgs a) {
     using W = WaveShape<KIND, M, Q>;
     constexpr int S = W::S, N = W::N, P = W::P, QP = W::QP;
     static_assert(N + 1 <= 64 && Q <= 64, "wave kernel needs n < 64 and Q <= 64");
@@ -169,6 +169,7 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
         const double ue = lane < N ? uu[ei * S + el] : 0.0;
 
         double Lk[S];  // Lambda at this lane's point for the last gradient
+        double uproj = 0.0;  // <U phiW> of this lane's dual at the last gradient
         // gradient at X (LDS); returns ||g||, leaves g_e in gout, Lambda in Lk
         auto gradient = [&](const double* X, double& gout) -> double {
             // phase: gradient
@@ -194,6 +195,7 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
             const double* wr = phiwT + ei * QP;
 #pragma unroll 7
             for (int k = 0; k < Q; ++k) acc += ur[k] * wr[k];
+            uproj = acc;
             double g = lane < N ? acc - ue : 0.0;
             gout = g;
             double nrm = sqrt(wave_sum(g * g));
@@ -321,10 +323,13 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
         auto cholesky_solve = [&](double g, double& d) -> bool {
             // phase: factor
             // lane c < N: column c of H; lane N: right-hand side.  Step j
-            // scales row j of U (lane r holds U(j, r)) and every lane reads it
-            // back from LDS as broadcasts (ds_read2_b64) for its rank-1 update;
-            // the row buffer alternates so a step's reads never race the next
-            // step's writes.
+            // scales row j of U (lane r holds U(j, r)); every lane needs all of
+            // it for its rank-1 update.  Half of the row reaches the lanes as
+            // LDS broadcasts (ds_read2_b64, the CU's one LDS pipe), half through
+            // v_readlane into SGPRs (each SIMD's own VALU): pairs alternate so
+            // neither pipe is the bottleneck, and U(j, j+1), on the path to the
+            // next pivot, always takes the SGPR route.  The row buffer alternates so a
+            // step's reads never race the next step's writes.
             double Hc[N];
             yb[lane] = g;
             wave_sync();
@@ -352,7 +357,11 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
                 urow[lane] = hj;
                 wave_sync();
 #pragma unroll
-                for (int r = j + 1; r < N; ++r) Hc[r] -= urow[r] * hj;
+                for (int r = j + 1; r < N; ++r) {
+                    // r = j + 1 feeds the next pivot: SGPR path, off the LDS latency
+                    const double ujr = (r == j + 1 || ((r - j) & 2)) ? readlane_f64(hj, r) : urow[r];
+                    Hc[r] -= ujr * hj;
+                }
             }
             if (!ok) return false;
             // phase end: factor
@@ -447,6 +456,17 @@ __global__ void __launch_bounds__(256) newton_wave_kernel(NewtonArgs a) {
         }
         if (status < 0) status = UQK_STATUS_MAX_ITERATIONS;
         for (int t = lane; t < SM; t += 64) lg[t] = lam[t];
+        if (a.uq_out) {
+            // the step's next two sweeps, from the last gradient at the
+            // returned lambda: uQ = U(lambda phi) (momentsolver.cpp:100) and
+            // u = <uQ phiW> (:120-124)
+            double* uqg = a.uq_out + cell * (long)Q * S;
+            for (int t = lane; t < Q * S; t += 64) {
+                const int k = t / S, s = t - k * S;
+                uqg[t] = Ub[s * QP + k];
+            }
+            if (lane < N) a.u_out[cell * SM + ei * S + el] = uproj;
+        }
         if (lane == 0) {
             if (a.status) a.status[cell] = status;
             if (a.iters) a.iters[cell] = iters;

@@ -13,7 +13,7 @@
 namespace uqk {
 
 constexpr int kSweepThreads = 256;
-constexpr int kPartialBlocks = 1024;
+constexpr int kPartialBlocks = 8192;  // grid of the grid-stride sweeps and size of the partials
 
 int sweep_partial_blocks(long) { return kPartialBlocks; }
 
@@ -129,6 +129,14 @@ __global__ void __launch_bounds__(kSweepThreads) advance_kernel(SweepArgs a) {
     const int wave = threadIdx.x >> 6;
     const int M = a.M, Q = a.Q, SM = S * M, E = a.nedge, D = a.dim;
     double* Gs = smem + wave * Q * S;
+    // the projection table PhiW in LDS when it fits (launcher's choice)
+    const bool wlds = a.table_lds != 0;
+    double* wt = smem + (blockDim.x >> 6) * Q * S;
+    if (wlds) {
+        for (int t = threadIdx.x; t < Q * M; t += blockDim.x) wt[t] = a.phiw[t];
+        __syncthreads();
+    }
+    const double* phiw = wlds ? wt : a.phiw;
     const double dt = a.dt_dev ? *a.dt_dev : a.dt;
     double res = 0.0;
     const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
@@ -201,13 +209,13 @@ __global__ void __launch_bounds__(kSweepThreads) advance_kernel(SweepArgs a) {
             int i = e / S, s = e - i * S;
             double rhs = 0.0, src = 0.0;
             if (i < nt) {
-                for (int k = 0; k < Q; ++k) rhs += Gs[k * S + s] * a.phiw[k * M + i];
+                for (int k = 0; k < Q; ++k) rhs += Gs[k * S + s] * phiw[k * M + i];
                 if (fric && s > 0) {
                     for (int k = 0; k < Q; ++k) {
                         const double* q = uqj + k * S;
                         double cf = a.p.friction * (1.0 + (a.p.friction_dim >= 0 ? a.xi[k * a.d + a.p.friction_dim] : 0.0));
                         double sp = sqrt(q[1] * q[1] + q[2] * q[2]) / q[0];
-                        src += -cf * sp * q[s] / q[0] * a.phiw[k * M + i];
+                        src += -cf * sp * q[s] / q[0] * phiw[k * M + i];
                     }
                 }
             }
@@ -341,7 +349,10 @@ hipError_t launch_dt(int kind, const SweepArgs& a, hipStream_t st) {
 template <int KIND> static void go_advance(const SweepArgs& a, hipStream_t st) {
     constexpr int S = KindInfo<KIND>::S;
     size_t shmem = sizeof(double) * (kSweepThreads / 64) * a.Q * S;
-    hipLaunchKernelGGL(advance_kernel<KIND>, dim3(kPartialBlocks), dim3(kSweepThreads), shmem, st, a);
+    SweepArgs b = a;
+    b.table_lds = a.Q * a.M <= 2048;  // PhiW up to 16 KB goes to LDS
+    if (b.table_lds) shmem += sizeof(double) * a.Q * a.M;
+    hipLaunchKernelGGL(advance_kernel<KIND>, dim3(kPartialBlocks), dim3(kSweepThreads), shmem, st, b);
 }
 hipError_t launch_advance(int kind, const SweepArgs& a, hipStream_t st) {
     UQK_DISPATCH(kind, go_advance, a, st);

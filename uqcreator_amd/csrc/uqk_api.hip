@@ -79,8 +79,12 @@ int expected_states(int kind) {
 }
 
 // Launch the Newton closure kernel; leaves the failed-cell count in d_counters[1].
+// When uq_out/u_out are given and the specialised kernel runs (not one-shot),
+// it also writes uQ and the reconstructed moments and *fused is set.
 int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, double* lam, int32_t* status,
-                   int32_t* iters, const int* active = nullptr) {
+                   int32_t* iters, const int* active = nullptr, double* uq_out = nullptr, double* u_out = nullptr,
+                   bool* fused = nullptr) {
+    if (fused) *fused = false;
     ctx->timed = false;
     UQK_HIP(ctx, uqk::launch_fill_u64(ctx->d_counters, 4, ctx->stream));  // [4..7] belong to the warm start
     if (n <= 0) return 0;
@@ -107,6 +111,9 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     const bool ev = !ctx->capturing;
     if (ctx->kernel != UQK_KERNEL_GENERIC) {
         hipError_t e = hipSuccess;
+        const bool fuse = uq_out && u_out && ctx->c.max_iter != 1;
+        a.uq_out = fuse ? uq_out : nullptr;
+        a.u_out = fuse ? u_out : nullptr;
         if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         if (uqk::newton_wave_launch(ctx->kind, ctx->M, ctx->Q, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream, &e,
                                     ctx->shape)) {
@@ -114,11 +121,14 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
             if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
             ctx->last_kernel = UQK_KERNEL_WAVE;
             ctx->timed = ev;
+            if (fused) *fused = fuse;
             return 0;
         }
         if (ctx->kernel == UQK_KERNEL_WAVE) return fail(ctx, -3, "no specialised closure kernel for this shape");
     }
     ctx->last_kernel = UQK_KERNEL_GENERIC;
+    a.uq_out = nullptr;
+    a.u_out = nullptr;
     a.wave_doubles = uqk::newton_fp64_wave_doubles(ctx->S, ctx->M, ctx->Q);
     const size_t wave_bytes = sizeof(double) * (size_t)a.wave_doubles;
     const size_t table_bytes = sizeof(double) * 2 * (size_t)ctx->Q * ctx->M;
@@ -159,7 +169,8 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
 int enqueue_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const int32_t* nt, double* u,
                  double* lam, double* uq, double* u_scratch, int32_t* status, int32_t* iters, const int* active) {
     const int64_t n = mesh->n_cells;
-    int rc = launch_closure(ctx, n, nt, u, lam, status, iters, active);
+    bool fused = false;
+    int rc = launch_closure(ctx, n, nt, u, lam, status, iters, active, uq, u_scratch, &fused);
     if (rc) return rc;
     uqk::SweepArgs a = sweep_args(ctx);
     a.p = problem_params(prob);
@@ -169,10 +180,10 @@ int enqueue_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, co
     a.lam = lam;
     a.uq_out = uq;
     a.active = active;
-    UQK_HIP(ctx, uqk::launch_eval_uq(ctx->kind, a, ctx->stream));
+    if (!fused) UQK_HIP(ctx, uqk::launch_eval_uq(ctx->kind, a, ctx->stream));
     a.uq = uq;
     a.u_out = u_scratch;
-    UQK_HIP(ctx, uqk::launch_reconstruct(a, ctx->stream));
+    if (!fused) UQK_HIP(ctx, uqk::launch_reconstruct(a, ctx->stream));
     a.scalar_out = ctx->d_scalars;
     UQK_HIP(ctx, uqk::launch_dt(ctx->kind, a, ctx->stream));
     a.u = u_scratch;

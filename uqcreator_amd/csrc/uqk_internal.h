@@ -27,6 +27,8 @@ struct NewtonArgs {
     int wave_doubles;            // LDS doubles per wave
     int prec;                    // UQK_FP64 or UQK_MIXED (Hessian arithmetic)
     const int* active;           // device flag: 0 skips the launch (graph replays past t_end)
+    double* uq_out;              // wave kernel, when not null: U at the quadrature points of the
+    double* u_out;               // returned lambda and its moments <U phiW> (fused uQ + reconstruction)
     long waves;                  // resident waves of the launch (work-queue chunking), set by the launcher
     int qbits;                   // work queue of 2^qbits positions >= ncell, set by the launcher
 };
@@ -99,6 +101,7 @@ struct SweepArgs {
     int npartial;
     double* scalar_out;          // final reduction result
     const int* active;           // device flag: 0 skips the launch
+    int table_lds;               // advance: PhiW staged in LDS (set by the launcher)
 };
 
 hipError_t launch_eval_uq(int kind, const SweepArgs& a, hipStream_t st);
