@@ -78,6 +78,13 @@ int expected_states(int kind) {
     return -1;
 }
 
+// The sweeps handle lines (2 edges, dim 1) and quadrilaterals (4 edges, dim 2).
+int check_mesh(uqk_ctx* ctx, const uqk_mesh* m) {
+    if (m->n_cells < 0 || m->n_edges < 1 || m->n_edges > 4 || m->dim < 1 || m->dim > 2)
+        return fail(ctx, -1, "mesh: n_edges must be 1..4 and dim 1 or 2");
+    return 0;
+}
+
 // Launch the Newton closure kernel; leaves the failed-cell count in d_counters[1].
 // When uq_out/u_out are given and the specialised kernel runs (not one-shot),
 // it also writes uQ and the reconstructed moments and *fused is set.
@@ -403,6 +410,7 @@ int uqk_reconstruct(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* uq
 
 int uqk_compute_dt(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const double* uq, double* dt_host) {
     if (!ctx || !prob || !mesh) return -1;
+    if (int rc = check_mesh(ctx, mesh)) return rc;
     uqk::SweepArgs a = sweep_args(ctx);
     a.p = problem_params(prob);
     set_mesh(a, mesh);
@@ -418,6 +426,7 @@ int uqk_compute_dt(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, 
 int uqk_advance(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const int32_t* nt, double dt,
                 const double* u, const double* uq, double* unew) {
     if (!ctx || !prob || !mesh) return -1;
+    if (int rc = check_mesh(ctx, mesh)) return rc;
     uqk::SweepArgs a = sweep_args(ctx);
     a.p = problem_params(prob);
     set_mesh(a, mesh);
@@ -436,6 +445,7 @@ int64_t uqk_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, co
                  double* lam, double* uq, double* u_scratch, int32_t* status, int32_t* iters, double* dt_out,
                  double* residual_out) {
     if (!ctx || !prob || !mesh) return -1;
+    if (int rc = check_mesh(ctx, mesh)) return rc;
     UQK_HIP(ctx, hipSetDevice(ctx->device));
     int rc = enqueue_step(ctx, prob, mesh, nt, u, lam, uq, u_scratch, status, iters, nullptr);
     if (rc) return rc;
@@ -452,6 +462,7 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
                       double t_end, int64_t max_steps, int32_t use_graph, double* t_out, int64_t* steps_out,
                       double* residual_out, int64_t* newton_out) {
     if (!ctx || !prob || !mesh) return -1;
+    if (int rc = check_mesh(ctx, mesh)) return rc;
     UQK_HIP(ctx, hipSetDevice(ctx->device));
     if (!ctx->d_run) UQK_HIP(ctx, hipMalloc(&ctx->d_run, sizeof(uqk::RunState)));
     uqk::RunState init{};
