@@ -25,6 +25,10 @@ HIP_FLAGS = [
     # MFMA accumulators in VGPRs (not AGPRs): the accumulator registers are
     # reused by the other phases, which keeps the closure kernels at 2+ waves/SIMD
     "-mllvm", "-amdgpu-mfma-vgpr-form",
+    # the closure's factorisation is one fully unrolled loop nest over
+    # register-resident tiles; the default limit would leave it rolled (and
+    # the tiles in scratch memory)
+    "-mllvm", "-pragma-unroll-threshold=500000",
     "-munsafe-fp-atomics", "-Wall", "-Wno-unused-result",
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
 ]

@@ -58,8 +58,8 @@ struct WaveShape {
     static constexpr int HS = UB + S * QP;       // packed H / U factor
     static constexpr int YB = HS + HP;           // 64
     static constexpr int DIAG = YB + 64;         // 64 reciprocal pivots
-    static constexpr int ROW = DIAG + 64;        // 2 x 64: row j of U, double buffered
-    static constexpr int WAVE_DOUBLES = ROW + 128;
+    static constexpr int ROW = DIAG + 64;        // 2 x 128: rows j, j+1 of U, double buffered
+    static constexpr int WAVE_DOUBLES = ROW + 256;
     static constexpr int TABLE_DOUBLES = 2 * M * QP;  // phiT, phiwT (i-major)
 };
 
@@ -322,60 +322,123 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
         // d <- H^{-1} g; returns false if H is not positive definite
         auto cholesky_solve = [&](double g, double& d) -> bool {
             // phase: factor
-            // lane c < N: column c of H; lane N: right-hand side.  Step j
-            // scales row j of U (lane r holds U(j, r)); every lane needs all of
-            // it for its rank-1 update.  Half of the row reaches the lanes as
-            // LDS broadcasts (ds_read2_b64, the CU's one LDS pipe), half through
-            // v_readlane into SGPRs (each SIMD's own VALU): pairs alternate so
-            // neither pipe is the bottleneck, and U(j, j+1), on the path to the
-            // next pivot, always takes the SGPR route.  The row buffer alternates so a
-            // step's reads never race the next step's writes.
-            double Hc[N];
+            // [H | g] (upper triangle and the right-hand side as column N) is
+            // spread over the wave in registers: lane 8p + q holds rows
+            // 16 ra + 2p + i (i = 0, 1) of columns 8b + q in T[ra][i][b], i.e.
+            // consecutive row pairs cyclically over the 8 row groups and
+            // columns cyclically over the 8 column groups.  A step eliminates
+            // two pivots: their 2x2 diagonal block is factored in scalar
+            // registers, the 8 owner lanes of the two rows scale them and
+            // publish them in an LDS row buffer, and every lane applies the
+            // rank-2 update to its tiles (the two rank-1 updates in order).
+            // Per lane this is ~n^3/(6*64) FMAs and one LDS round trip per two
+            // pivots.  Elimination of column N is the forward substitution.
+            constexpr int NS = (N + 1 + 7) / 8;   // column slots
+            constexpr int RA = (N + 1 + 15) / 16; // row-pair slots
+            const int p8 = lane >> 3, q8 = lane & 7;
+            double T[RA][2][NS];
             yb[lane] = g;
             wave_sync();
-            if (lane < N) {
-                const int base = lane * (lane + 1) / 2;
 #pragma unroll
-                for (int r = 0; r < N; ++r) Hc[r] = (r <= lane) ? Hs[base + (r <= lane ? r : 0)] : 0.0;
-            } else {
+            for (int ra = 0; ra < RA; ++ra)
 #pragma unroll
-                for (int r = 0; r < N; ++r) Hc[r] = yb[r];
-            }
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int tb = 0; tb < NS; ++tb) {
+                        const int r = 16 * ra + 2 * p8 + i, cc = 8 * tb + q8;
+                        double v = 0.0;
+                        if (r <= cc && cc < N) v = Hs[cc * (cc + 1) / 2 + r];
+                        else if (cc == N && r < N) v = yb[r];
+                        T[ra][i][tb] = v;
+                    }
             bool ok = true;
 #pragma unroll
-            for (int j = 0; j < N; ++j) {
-                const double hjj = readlane_f64(Hc[j], j);
-                if (!(hjj > 0.0)) {
+            for (int j = 0; j < N; j += 2) {
+                const bool two = j + 1 < N;          // the last step of an odd n has one pivot
+                const int pj = (j & 15) >> 1, aj = j >> 4;
+                const int l0 = 8 * pj + (j & 7), l1 = 8 * pj + ((j + 1) & 7);
+                const double h00 = readlane_f64(T[aj][0][j >> 3], l0);
+                if (!(h00 > 0.0)) {
                     ok = false;
                     break;
                 }
-                const double rinv = rsqrt_f64(hjj);
-                if (lane == 0) rdiag[j] = rinv;
-                Hc[j] = (lane == j) ? hjj * rinv : Hc[j] * rinv;
-                const double hj = Hc[j];
-                double* urow = rowb + 64 * (j & 1);
-                urow[lane] = hj;
-                wave_sync();
-#pragma unroll
-                for (int r = j + 1; r < N; ++r) {
-                    // r = j + 1 feeds the next pivot: SGPR path, off the LDS latency
-                    const double ujr = (r == j + 1 || ((r - j) & 2)) ? readlane_f64(hj, r) : urow[r];
-                    Hc[r] -= ujr * hj;
+                const double r0 = rsqrt_f64(h00);
+                const double u00 = h00 * r0;
+                double r1 = 0.0, u01 = 0.0, u11 = 0.0;
+                if (two) {
+                    u01 = readlane_f64(T[aj][0][(j + 1) >> 3], l1) * r0;
+                    const double h11 = readlane_f64(T[aj][1][(j + 1) >> 3], l1) - u01 * u01;
+                    if (!(h11 > 0.0)) {
+                        ok = false;
+                        break;
+                    }
+                    r1 = rsqrt_f64(h11);
+                    u11 = h11 * r1;
                 }
+                if (lane == 0) {
+                    rdiag[j] = r0;
+                    if (two) rdiag[j + 1] = r1;
+                }
+                double* urow = rowb + 128 * ((j >> 1) & 1);  // rows j, j+1 at [c], [64 + c]
+                if (p8 == pj) {
+#pragma unroll
+                    for (int tb = j >> 3; tb < NS; ++tb) {
+                        const int cc = 8 * tb + q8;
+                        if (cc >= j) {
+                            const double v0 = (cc == j) ? u00 : T[aj][0][tb] * r0;
+                            T[aj][0][tb] = v0;
+                            urow[cc] = v0;
+                            if (two && cc > j) {
+                                const double v1 = (cc == j + 1) ? u11 : (T[aj][1][tb] - u01 * v0) * r1;
+                                T[aj][1][tb] = v1;
+                                urow[64 + cc] = v1;
+                            }
+                        }
+                    }
+                }
+                wave_sync();
+                double ur0[RA][2], ur1[RA][2], uc0[NS], uc1[NS];
+#pragma unroll
+                for (int ra = aj; ra < RA; ++ra)
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) {
+                        ur0[ra][i] = urow[16 * ra + 2 * p8 + i];
+                        if (two) ur1[ra][i] = urow[64 + 16 * ra + 2 * p8 + i];
+                    }
+#pragma unroll
+                for (int tb = j >> 3; tb < NS; ++tb) {
+                    uc0[tb] = urow[8 * tb + q8];
+                    if (two) uc1[tb] = urow[64 + 8 * tb + q8];
+                }
+                const int jl = two ? j + 1 : j;  // last eliminated pivot
+#pragma unroll
+                for (int ra = aj; ra < RA; ++ra)
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+#pragma unroll
+                        for (int tb = j >> 3; tb < NS; ++tb) {
+                            const int r = 16 * ra + 2 * p8 + i, cc = 8 * tb + q8;
+                            if (r > jl && cc > jl && r <= cc && r < N && cc <= N) {
+                                double t = T[ra][i][tb] - ur0[ra][i] * uc0[tb];
+                                if (two) t -= ur1[ra][i] * uc1[tb];
+                                T[ra][i][tb] = t;
+                            }
+                        }
             }
             if (!ok) return false;
             // phase end: factor
             // phase: backsub
-            // U columns -> Hs (packed), y -> yb
-            if (lane < N) {
-                const int base = lane * (lane + 1) / 2;
+            // U -> Hs (packed), y -> yb
 #pragma unroll
-                for (int r = 0; r < N; ++r)
-                    if (r <= lane) Hs[base + r] = Hc[r];
-            } else if (lane == N) {
+            for (int ra = 0; ra < RA; ++ra)
 #pragma unroll
-                for (int r = 0; r < N; ++r) yb[r] = Hc[r];
-            }
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int tb = 0; tb < NS; ++tb) {
+                        const int r = 16 * ra + 2 * p8 + i, cc = 8 * tb + q8;
+                        if (r <= cc && cc < N) Hs[cc * (cc + 1) / 2 + r] = T[ra][i][tb];
+                        else if (cc == N && r < N) yb[r] = T[ra][i][tb];
+                    }
             wave_sync();
             double yr = lane < N ? yb[lane] : 0.0;
 #pragma unroll
