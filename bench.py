@@ -173,7 +173,7 @@ def main():
     ap.add_argument("--no-warm-start", action="store_true")
     ap.add_argument("--train-cells", type=int, default=1 << 17)
     ap.add_argument("--train-steps", type=int, default=2400)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 14)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 17)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", choices=["auto", "generic", "wave"], default="auto")
     args = ap.parse_args()

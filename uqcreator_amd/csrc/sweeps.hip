@@ -231,42 +231,61 @@ __global__ void __launch_bounds__(kSweepThreads) advance_kernel(SweepArgs a) {
     if (threadIdx.x == 0) a.partial[blockIdx.x] = res;
 }
 
-// One thread per cell: E and Var of each state from the closure at lambda.
+// E and Var of each state from the closure at lambda (WriteMeanAndVar,
+// momentsolver.cpp:302-328): one wavefront per cell, lanes over quadrature
+// points (up to 128 per cell), the two weighted sums as wave reductions.  The
+// cell's duals are read by all lanes at once (one cache line, broadcast).
 template <int KIND>
 __global__ void __launch_bounds__(kSweepThreads) mean_var_kernel(SweepArgs a) {
     constexpr int S = KindInfo<KIND>::S;
-    long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= a.ncell) return;
-    int nt = a.nt_cell ? a.nt_cell[cell] : a.M;
-    const double* lam = a.lam + cell * S * a.M;
-    double E[S], V[S], L[S], Uv[S];
+    constexpr int KP = 2;  // quadrature points per lane
+    const int lane = threadIdx.x & 63;
+    const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
+    for (long cell = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); cell < a.ncell; cell += nwaves) {
+        const int nt = a.nt_cell ? a.nt_cell[cell] : a.M;
+        const double* lam = a.lam + cell * S * a.M;
+        double Uv[KP][S], w0[KP];
 #pragma unroll
-    for (int s = 0; s < S; ++s) E[s] = V[s] = 0.0;
-    for (int pass = 0; pass < 2; ++pass) {
-        for (int k = 0; k < a.Q; ++k) {
+        for (int h = 0; h < KP; ++h) {
+            const int k = lane + 64 * h;
+            double L[S];
 #pragma unroll
             for (int s = 0; s < S; ++s) L[s] = 0.0;
-            for (int i = 0; i < nt; ++i) {
-                double p = a.phi[k * a.M + i];
+            w0[h] = 0.0;
+            if (k < a.Q) {
+                for (int i = 0; i < nt; ++i) {
+                    const double p = a.phi[k * a.M + i];
 #pragma unroll
-                for (int s = 0; s < S; ++s) L[s] += lam[i * S + s] * p;
+                    for (int s = 0; s < S; ++s) L[s] += lam[i * S + s] * p;
+                }
+                w0[h] = a.phiw[k * a.M + 0];
             }
-            closure_U<KIND>(a.c, L, Uv);
-            double w0 = a.phiw[k * a.M + 0];
+            closure_U<KIND>(a.c, L, Uv[h]);
+            if (k >= a.Q) {
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                if (pass == 0)
-                    E[s] += Uv[s] * w0;
-                else
-                    V[s] += (Uv[s] - E[s]) * (Uv[s] - E[s]) * w0;
+                for (int s = 0; s < S; ++s) Uv[h][s] = 0.0;  // U of Lambda = 0 may be NaN
             }
         }
-    }
-    double* out = a.mv_out + cell * 2 * S;
+        double E[S];
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-        out[s] = E[s];
-        out[S + s] = V[s];
+        for (int s = 0; s < S; ++s) {
+            double e = 0.0;
+#pragma unroll
+            for (int h = 0; h < KP; ++h) e += Uv[h][s] * w0[h];
+            E[s] = wave_sum(e);
+        }
+        double* out = a.mv_out + cell * 2 * S;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            double v = 0.0;
+#pragma unroll
+            for (int h = 0; h < KP; ++h) v += (Uv[h][s] - E[s]) * (Uv[h][s] - E[s]) * w0[h];
+            v = wave_sum(v);
+            if (lane == 0) {
+                out[s] = E[s];
+                out[S + s] = v;
+            }
+        }
     }
 }
 
@@ -367,11 +386,14 @@ hipError_t launch_residual(const SweepArgs& a, hipStream_t st) {
 }
 
 template <int KIND> static void go_mean_var(const SweepArgs& a, hipStream_t st) {
-    int blocks = (int)((a.ncell + kSweepThreads - 1) / kSweepThreads);
+    const long waves = kSweepThreads / 64;
+    const long want = (a.ncell + waves - 1) / waves;
+    const int blocks = (int)(want < kPartialBlocks ? want : kPartialBlocks);
     hipLaunchKernelGGL(mean_var_kernel<KIND>, dim3(blocks), dim3(kSweepThreads), 0, st, a);
 }
 hipError_t launch_mean_var(int kind, const SweepArgs& a, hipStream_t st) {
     if (a.ncell <= 0) return hipSuccess;
+    if (a.Q > 128) return hipErrorInvalidValue;  // two quadrature points per lane
     UQK_DISPATCH(kind, go_mean_var, a, st);
     return hipGetLastError();
 }
