@@ -59,3 +59,25 @@ def test_euler2d_input_short_run(oracle, tmp_path, precision, tol):
     print(precision, "report", rep, "rel L2 E/Var", err, flush=True)
     assert rep["steps"] == 10
     assert err.max() < tol
+
+
+def test_shallow_water_input_short_run(oracle, tmp_path):
+    """The dam-break input (barrier closure, n = 105, 5^3 points, bed friction
+    on the third uncertain dimension) on a 16 x 16 grid, 4 steps, against the
+    oracle's time loop."""
+    text = open(os.path.join(INPUT, "shallowwater2d_dambreak.toml")).read()
+    for a, b in [("nx = 1024", "nx = 16"), ("ny = 1024", "ny = 16"), ("maxSteps = 500", "maxSteps = 4"),
+                 ('precision = "mixed"', 'precision = "fp64"')]:
+        assert a in text
+        text = text.replace(a, b)
+    path = tmp_path / "sw.toml"
+    path.write_text(text)
+    rep, err = _run(oracle, str(path), tmp_path, native.Basis(3, [4], 5),
+                    meshes.quad_grid(16, 16, -1.0, 1.0, -1.0, 1.0, native.BC_DIRICHLET),
+                    native.SHALLOW_WATER2D_BARRIER,
+                    dict(kind=native.SHALLOW_WATER2D_PROBLEM, flux=native.FLUX_REFERENCE, cfl=0.45, friction=0.01,
+                         friction_dim=2),
+                    dict(barrier=0.01, eps=1e-8), 1e3, 4)
+    print("report", rep, "rel L2 E/Var", err, flush=True)
+    assert rep["steps"] == 4
+    assert err.max() < 1e-8

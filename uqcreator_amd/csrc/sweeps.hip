@@ -128,10 +128,10 @@ __global__ void __launch_bounds__(kSweepThreads) advance_kernel(SweepArgs a) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int M = a.M, Q = a.Q, SM = S * M, E = a.nedge, D = a.dim;
-    double* Gs = smem + wave * Q * S;
+    double* Gs = smem + wave * E * Q * S;  // point fluxes per (edge, point)
     // the projection table PhiW in LDS when it fits (launcher's choice)
     const bool wlds = a.table_lds != 0;
-    double* wt = smem + (blockDim.x >> 6) * Q * S;
+    double* wt = smem + (blockDim.x >> 6) * E * Q * S;
     if (wlds) {
         for (int t = threadIdx.x; t < Q * M; t += blockDim.x) wt[t] = a.phiw[t];
         __syncthreads();
@@ -153,54 +153,56 @@ __global__ void __launch_bounds__(kSweepThreads) advance_kernel(SweepArgs a) {
         const int nt = a.nt_cell ? a.nt_cell[cell] : M;
         const double* uqj = a.uq + cell * S * Q;
         const bool wall = (bc == UQK_BC_NOSLIP || bc == UQK_BC_SWWALL);
-        for (int k = lane; k < Q; k += 64) {
-            double uj[S], gsum[S], G[S], gh[S], nbv[S];
+        // one point flux per lane and item (edge e, point k): every lane's
+        // loads are independent, so a cell's edges overlap their latencies
+        for (int t = lane; t < E * Q; t += 64) {
+            const int e = t / Q, k = t - e * Q;
+            const int nb = a.nbr[cell * E + e];
+            const double* nu = a.unit_normal + (cell * E + e) * D;
+            const double* nv = a.normal + (cell * E + e) * D;
+            const double nux = nu[0], nuy = D > 1 ? nu[1] : 0.0, nx = nv[0], ny = D > 1 ? nv[1] : 0.0;
+            double uj[S], G[S], v[S];
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                uj[s] = uqj[k * S + s];
-                gsum[s] = 0.0;
-            }
-            if constexpr (S >= 3) {
-                if (wall) {
+            for (int s = 0; s < S; ++s) uj[s] = uqj[k * S + s];
+            if (wall && nb == a.ncell) {
+                if constexpr (S >= 3) {
                     // ghost state: velocity reflected to the tangential part
-                    // (expliciteuler.cpp:25-58)
-                    double bx = a.bnormal[cell * D + 0], by = a.bnormal[cell * D + 1];
+                    // (expliciteuler.cpp:25-58); the flux is G(ghost, ghost)
+                    const double bx = a.bnormal[cell * D + 0], by = a.bnormal[cell * D + 1];
+                    const double rho = uj[0], vx = uj[1] / rho, vy = uj[2] / rho;
+                    const double vn = bx * vx + by * vy;
+                    const double tx = vx - vn * bx, ty = vy - vn * by;
 #pragma unroll
-                    for (int s = 0; s < S; ++s) gh[s] = uj[s];
-                    double vx = gh[1] / gh[0], vy = gh[2] / gh[0];
-                    double vn = bx * vx + by * vy;
-                    double tx = vx - vn * bx, ty = vy - vn * by;
-                    double rho = gh[0];
+                    for (int s = 0; s < S; ++s) v[s] = uj[s];
                     if constexpr (S == 4) {
-                        if (bc == UQK_BC_NOSLIP) gh[3] += rho * 0.5 * ((tx * tx + ty * ty) - (vx * vx + vy * vy));
+                        if (bc == UQK_BC_NOSLIP) v[3] += rho * 0.5 * ((tx * tx + ty * ty) - (vx * vx + vy * vy));
                     }
-                    gh[1] = rho * tx;
-                    gh[2] = rho * ty;
-                }
-            }
-            for (int e = 0; e < E; ++e) {
-                const int nb = a.nbr[cell * E + e];
-                const double* nu = a.unit_normal + (cell * E + e) * D;
-                const double* nv = a.normal + (cell * E + e) * D;
-                double nux = nu[0], nuy = D > 1 ? nu[1] : 0.0, nx = nv[0], ny = D > 1 ? nv[1] : 0.0;
-                if (wall && nb == a.ncell) {
-                    point_flux<S>(a.p, gh, gh, nux, nuy, nx, ny, G);
+                    v[1] = rho * tx;
+                    v[2] = rho * ty;
+#pragma unroll
+                    for (int s = 0; s < S; ++s) uj[s] = v[s];
                 } else {
-                    if (nb == a.ncell) {
-                        // the reference reads the never-written uQ[nCells] (zero)
 #pragma unroll
-                        for (int s = 0; s < S; ++s) nbv[s] = 0.0;
-                    } else {
-#pragma unroll
-                        for (int s = 0; s < S; ++s) nbv[s] = a.uq[((long)nb * Q + k) * S + s];
-                    }
-                    point_flux<S>(a.p, uj, nbv, nux, nuy, nx, ny, G);
+                    for (int s = 0; s < S; ++s) v[s] = uj[s];
                 }
+            } else if (nb == a.ncell) {
+                // the reference reads the never-written uQ[nCells] (zero)
 #pragma unroll
-                for (int s = 0; s < S; ++s) gsum[s] += G[s];
+                for (int s = 0; s < S; ++s) v[s] = 0.0;
+            } else {
+#pragma unroll
+                for (int s = 0; s < S; ++s) v[s] = a.uq[((long)nb * Q + k) * S + s];
             }
+            point_flux<S>(a.p, uj, v, nux, nuy, nx, ny, G);
 #pragma unroll
-            for (int s = 0; s < S; ++s) Gs[k * S + s] = gsum[s];
+            for (int s = 0; s < S; ++s) Gs[t * S + s] = G[s];
+        }
+        wave_sync();
+        // sum over the edges in edge order, in place into the first edge's slots
+        for (int t = lane; t < Q * S; t += 64) {
+            double sum = 0.0;
+            for (int e = 0; e < E; ++e) sum += Gs[e * Q * S + t];
+            Gs[t] = sum;
         }
         wave_sync();
         const double f = dt / a.area[cell];
@@ -367,7 +369,7 @@ hipError_t launch_dt(int kind, const SweepArgs& a, hipStream_t st) {
 
 template <int KIND> static void go_advance(const SweepArgs& a, hipStream_t st) {
     constexpr int S = KindInfo<KIND>::S;
-    size_t shmem = sizeof(double) * (kSweepThreads / 64) * a.Q * S;
+    size_t shmem = sizeof(double) * (kSweepThreads / 64) * a.nedge * a.Q * S;
     SweepArgs b = a;
     b.table_lds = a.Q * a.M <= 2048;  // PhiW up to 16 KB goes to LDS
     if (b.table_lds) shmem += sizeof(double) * a.Q * a.M;
