@@ -118,10 +118,11 @@ int uqk_synchronize(uqk_ctx* ctx);
 
 /* Closure kernel selection: AUTO picks the compile-time specialised wave
  * kernel when one exists for (closure, M, Q) and the batch has one order,
- * else the generic kernel (any size, per-cell orders). */
-enum { UQK_KERNEL_AUTO = 0, UQK_KERNEL_GENERIC = 1, UQK_KERNEL_WAVE = 2 };
+ * else the workgroup-per-cell kernel for systems of 64..256 duals, else the
+ * generic wave-per-cell kernel (any size, per-cell orders). */
+enum { UQK_KERNEL_AUTO = 0, UQK_KERNEL_GENERIC = 1, UQK_KERNEL_WAVE = 2, UQK_KERNEL_BLOCK = 3 };
 int uqk_set_closure_kernel(uqk_ctx* ctx, int32_t kernel);
-/* Kernel used by the last closure call (UQK_KERNEL_GENERIC / UQK_KERNEL_WAVE). */
+/* Kernel used by the last closure call (UQK_KERNEL_GENERIC / _WAVE / _BLOCK). */
 int32_t uqk_last_closure_kernel(const uqk_ctx* ctx);
 
 /* Closure::SolveClosure / SolveClosureSafe (closure.cpp:133-193, :297-339),

@@ -116,7 +116,7 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     a.prec = ctx->precision;
     a.active = active;
     const bool ev = !ctx->capturing;
-    if (ctx->kernel != UQK_KERNEL_GENERIC) {
+    if (ctx->kernel == UQK_KERNEL_AUTO || ctx->kernel == UQK_KERNEL_WAVE) {
         hipError_t e = hipSuccess;
         const bool fuse = uq_out && u_out && ctx->c.max_iter != 1;
         a.uq_out = fuse ? uq_out : nullptr;
@@ -132,6 +132,18 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
             return 0;
         }
         if (ctx->kernel == UQK_KERNEL_WAVE) return fail(ctx, -3, "no specialised closure kernel for this shape");
+    }
+    const bool block_ok = uqk::newton_block_fits(ctx->S, ctx->M, ctx->Q, (int)ctx->max_lds);
+    if (ctx->kernel == UQK_KERNEL_BLOCK || (ctx->kernel == UQK_KERNEL_AUTO && ctx->S * ctx->M >= 64 && block_ok)) {
+        if (!block_ok) return fail(ctx, -3, "closure too large for the workgroup kernel");
+        a.uq_out = nullptr;
+        a.u_out = nullptr;
+        if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        UQK_HIP(ctx, uqk::newton_block_launch(ctx->kind, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream));
+        if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        ctx->last_kernel = UQK_KERNEL_BLOCK;
+        ctx->timed = ev;
+        return 0;
     }
     ctx->last_kernel = UQK_KERNEL_GENERIC;
     a.uq_out = nullptr;
@@ -310,7 +322,7 @@ int uqk_set_stream(uqk_ctx* ctx, void* s) {
 
 int uqk_set_closure_kernel(uqk_ctx* ctx, int32_t kernel) {
     if (!ctx) return -1;
-    if (kernel < UQK_KERNEL_AUTO || kernel > UQK_KERNEL_WAVE) return fail(ctx, -1, "unknown closure kernel");
+    if (kernel < UQK_KERNEL_AUTO || kernel > UQK_KERNEL_BLOCK) return fail(ctx, -1, "unknown closure kernel");
     ctx->kernel = kernel;
     return 0;
 }

@@ -66,6 +66,9 @@ bool newton_wave_launch(int kind, int M, int Q, const NewtonArgs& a, int max_lds
 // there is none.
 bool newton_wave_limits(int kind, int M, int Q, int prec, int max_lds, int waves, int* max_waves, int* occupancy);
 int newton_fp64_wave_doubles(int S, int M, int Q);
+// workgroup-per-cell kernel for n = S*M up to 256 (closure_block.hip)
+bool newton_block_fits(int S, int M, int Q, int max_lds);
+hipError_t newton_block_launch(int kind, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st);
 int newton_fp64_occupancy(int kind, bool tlds, int threads, size_t shmem);
 hipError_t newton_fp64_launch(int kind, const NewtonArgs& a, bool tlds, int blocks, int threads, size_t shmem,
                               hipStream_t st);

@@ -17,7 +17,7 @@ FP64, MIXED = 0, 1
 BURGERS, EULER2D_PROBLEM, SHALLOW_WATER2D_PROBLEM = range(3)
 FLUX_REFERENCE, FLUX_LAX_FRIEDRICHS = 0, 1
 BC_NONE, BC_DIRICHLET, BC_NOSLIP, BC_SWWALL = range(4)
-KERNEL_AUTO, KERNEL_GENERIC, KERNEL_WAVE = range(3)
+KERNEL_AUTO, KERNEL_GENERIC, KERNEL_WAVE, KERNEL_BLOCK = range(4)
 STATES = {BOUNDED_BARRIER: 1, EULER1D: 3, EULER2D: 4, SHALLOW_WATER2D: 3, SHALLOW_WATER2D_BARRIER: 3}
 # Newton damping per closure (boundedbarrier.cpp:4, eulerclosure.cpp:3,
 # eulerclosure2d.cpp:3, shallowwaterclosure2d.cpp:3-6)
