@@ -142,6 +142,21 @@ int uqk_eval_uq(uqk_ctx* ctx, int64_t n_cells, const int32_t* ntotal_cell, const
 /* u_j = uQ_j * phiw over the cell's nTotal moments, rest zero
  * (momentsolver.cpp:120-124, vectorspace.cpp:174-185).  Device pointers. */
 int uqk_reconstruct(uqk_ctx* ctx, int64_t n_cells, const int32_t* ntotal_cell, const double* uq, double* u);
+/* Adaptive refinement levels (momentsolver.cpp:104-117 with
+ * ComputeRefIndicator, :271-289).  ntotal_per_level: nTotal of each of the
+ * n_levels levels, ascending, within 1..n_moments.  A cell at level l has
+ * the indicator |u(0,nt-1)| + |u(0,nt-2)| for one uncertain dimension, else
+ * sum |u(0,i)| over the moments level l adds (from nTotal(l-1), or 1 at level
+ * 0); it moves up a level above refine_above and down below coarsen_below (the
+ * reference hard-codes 0.02 and 0.004).  With n_levels > 1, uqk_step and
+ * uqk_run_steps refine every step between uQ and the reconstruction, and
+ * update ntotal_cell (then required) in place; n_levels <= 1 turns it off. */
+int uqk_set_refinement(uqk_ctx* ctx, int32_t n_levels, const int32_t* ntotal_per_level, double refine_above,
+                       double coarsen_below);
+/* One refinement pass over n_cells: ntotal_cell (device, in/out) from the
+ * moments u (device).  Cells whose ntotal_cell is no level's nTotal are left
+ * unchanged.  Synchronous. */
+int uqk_refine(uqk_ctx* ctx, int64_t n_cells, const double* u, int32_t* ntotal_cell);
 /* dt = min_j Problem::ComputeDt(uQ_j, maxEdge_j/area_j) (momentsolver.cpp:127-133).
  * Result written to *dt_host. */
 int uqk_compute_dt(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const double* uq, double* dt_host);
@@ -150,10 +165,11 @@ int uqk_compute_dt(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, 
 int uqk_advance(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const int32_t* ntotal_cell,
                 double dt, const double* u, const double* uq, double* unew);
 /* One MomentSolver::Solve step body (momentsolver.cpp:83-152): closure, uQ,
- * reconstruction, dt, advance, residual.  u, lambda updated in place (device).
+ * reconstruction, dt, advance, residual.  u, lambda updated in place (device);
+ * ntotal_cell too when refinement levels are set (uqk_set_refinement).
  * Writes dt and the residual of the state-0 mean.  Returns the number of
  * cells whose closure failed (>= 0) or an error code. */
-int64_t uqk_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const int32_t* ntotal_cell,
+int64_t uqk_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, int32_t* ntotal_cell,
                  double* u, double* lambda, double* uq, double* u_scratch, int32_t* status, int32_t* iters,
                  double* dt_out, double* residual_out);
 /* The time loop of MomentSolver::Solve (momentsolver.cpp:79-162) on the
@@ -163,7 +179,7 @@ int64_t uqk_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, co
  * replayed; t, the step count and failures live on the device and are read
  * back every 64 steps, so there is no host synchronisation inside a step.
  * Returns the number of failed cells of the last step (>= 0) or an error. */
-int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const int32_t* ntotal_cell,
+int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, int32_t* ntotal_cell,
                       double* u, double* lambda, double* uq, double* u_scratch, int32_t* status, int32_t* iters,
                       double t0, double t_end, int64_t max_steps, int32_t use_graph, double* t_out, int64_t* steps_out,
                       double* residual_out, int64_t* newton_steps_out);

@@ -888,12 +888,54 @@ extern "C" int orc_advance(const orc_problem* p, const orc_mesh* mesh, int S, in
     return 0;
 }
 
-/* momentsolver.cpp:79-162 on one rank: closure -> uQ -> reconstruct -> dt ->
- * t += dt -> Advance -> residual of the state-0 mean. */
-extern "C" int orc_run(const orc_closure* c, const orc_problem* p, const orc_mesh* mesh, int M, int Q, int d,
-                       const double* phi, const double* phiw, const double* xi, const int* nt_cell, double* u,
-                       double* lam, double t_end, long max_steps, double* t_out, long* steps_out,
-                       double* residual_out, long* newton_iters_out, int nthreads) {
+/* MomentSolver::ComputeRefIndicator (momentsolver.cpp:271-289) for one
+ * cell's moments u ((s, i) at i*S + s) at level lev: one uncertain dimension
+ * takes |u(0, nt-1)| + |u(0, nt-2)|, more take the sum of |u(0, i)| for i from
+ * nTotal(lev-1) (1 at level 0) to nTotal(lev)-1. */
+extern "C" double orc_ref_indicator(int S, int d, const int* nt_level, int lev, const double* u) {
+    const int nt = nt_level[lev];
+    double ind = 0.0;
+    if (d == 1) {
+        ind = std::fabs(u[(size_t)(nt - 1) * S]);
+        if (nt >= 2) ind += std::fabs(u[(size_t)(nt - 2) * S]);
+        return ind;
+    }
+    const int i0 = lev == 0 ? 1 : nt_level[lev - 1];
+    for (int i = i0; i < nt; ++i) ind += std::fabs(u[(size_t)i * S]);
+    return ind;
+}
+
+/* The refinement loop of MomentSolver::Solve (momentsolver.cpp:104-117):
+ * up one level above `above` (0.02 there) unless finest, else down one below
+ * `below` (0.004) unless coarsest.  The level of a cell is the one whose
+ * nTotal equals nt_cell[j]; nt_cell is rewritten.  Cells whose nt_cell is no
+ * level's nTotal are left as they are. */
+extern "C" void orc_refine(int S, int M, int d, int nlev, const int* nt_level, double above, double below, long ncell,
+                           const double* u, int* nt_cell) {
+    for (long j = 0; j < ncell; ++j) {
+        int lev = -1;
+        for (int l = 0; l < nlev; ++l)
+            if (nt_level[l] == nt_cell[j]) lev = l;
+        if (lev < 0) continue;
+        const double ind = orc_ref_indicator(S, d, nt_level, lev, u + (size_t)j * S * M);
+        if (ind > above && lev < nlev - 1)
+            ++lev;
+        else if (ind < below && lev > 0)
+            --lev;
+        nt_cell[j] = nt_level[lev];
+    }
+}
+
+/* momentsolver.cpp:79-162 on one rank: closure -> uQ -> [refinement levels]
+ * -> reconstruct -> dt -> t += dt -> Advance -> residual of the state-0 mean.
+ * With nlev > 1 the levels move between uQ and the reconstruction
+ * (momentsolver.cpp:99-124): the closure and uQ use the old nTotal, the
+ * reconstruction and Advance the new one. */
+static int run_impl(const orc_closure* c, const orc_problem* p, const orc_mesh* mesh, int M, int Q, int d,
+                    const double* phi, const double* phiw, const double* xi, int* nt_cell, int nlev,
+                    const int* nt_level, double above, double below, double* u, double* lam, double t_end,
+                    long max_steps, double* t_out, long* steps_out, double* residual_out, long* newton_iters_out,
+                    int nthreads) {
     const int S = c->S;
     const long N = mesh->ncell;
     std::vector<double> uq((size_t)N * S * Q), unew((size_t)N * S * M), uold0(N);
@@ -910,6 +952,7 @@ extern "C" int orc_run(const orc_closure* c, const orc_problem* p, const orc_mes
         }
         for (long j = 0; j < N; ++j) uold0[j] = u[(size_t)j * S * M];
         orc_eval_uq(c, M, Q, phi, N, nt_cell, lam, uq.data(), nthreads);
+        if (nlev > 1) orc_refine(S, M, d, nlev, nt_level, above, below, N, u, nt_cell);
         orc_reconstruct(S, M, Q, phiw, N, nt_cell, uq.data(), u, nthreads);
         double dt = 1e10;
         for (long j = 0; j < N; ++j) dt = std::min(dt, orc_compute_dt(p, S, Q, uq.data() + (size_t)j * S * Q, mesh->dx_param[j]));
@@ -925,6 +968,24 @@ extern "C" int orc_run(const orc_closure* c, const orc_problem* p, const orc_mes
     if (residual_out) *residual_out = residual;
     if (newton_iters_out) *newton_iters_out = newton;
     return rc;
+}
+
+extern "C" int orc_run(const orc_closure* c, const orc_problem* p, const orc_mesh* mesh, int M, int Q, int d,
+                       const double* phi, const double* phiw, const double* xi, const int* nt_cell, double* u,
+                       double* lam, double t_end, long max_steps, double* t_out, long* steps_out,
+                       double* residual_out, long* newton_iters_out, int nthreads) {
+    return run_impl(c, p, mesh, M, Q, d, phi, phiw, xi, const_cast<int*>(nt_cell), 0, nullptr, 0.0, 0.0, u, lam,
+                    t_end, max_steps, t_out, steps_out, residual_out, newton_iters_out, nthreads);
+}
+
+/* The same loop with refinement levels; nt_cell (required) is updated. */
+extern "C" int orc_run_levels(const orc_closure* c, const orc_problem* p, const orc_mesh* mesh, int M, int Q, int d,
+                              const double* phi, const double* phiw, const double* xi, int nlev, const int* nt_level,
+                              double above, double below, int* nt_cell, double* u, double* lam, double t_end,
+                              long max_steps, double* t_out, long* steps_out, double* residual_out,
+                              long* newton_iters_out, int nthreads) {
+    return run_impl(c, p, mesh, M, Q, d, phi, phiw, xi, nt_cell, nlev, nt_level, above, below, u, lam, t_end,
+                    max_steps, t_out, steps_out, residual_out, newton_iters_out, nthreads);
 }
 
 /* momentsolver.cpp:687-749: area-weighted relative error per field.

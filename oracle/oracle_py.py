@@ -67,6 +67,7 @@ def lib():
         L.orc_mlp_loss_grad.restype = C.c_double
         L.orc_compute_dt.restype = C.c_double
         L.orc_compute_dt.argtypes = [C.POINTER(Problem), C.c_int, C.c_int, C.c_void_p, C.c_double]
+        L.orc_ref_indicator.restype = C.c_double
     return _lib
 
 
@@ -271,6 +272,41 @@ def run(c, p, mesh, phi, phiw, xi, u, lam, t_end, max_steps=-1, nt_cell=None, nt
     rc = lib().orc_run(C.byref(c), C.byref(p), C.byref(mesh.struct), M, Q, d, _p(phi), _p(phiw),
                        _p(np.ascontiguousarray(xi)), _p(ntc), _p(u), _p(lam), C.c_double(t_end), C.c_long(max_steps),
                        C.byref(t), C.byref(steps), C.byref(res), C.byref(newton), nthreads)
+    return dict(rc=rc, t=t.value, steps=steps.value, residual=res.value, newton_iters=newton.value)
+
+
+def ref_indicator(S, d, nt_level, lev, u_cell):
+    """momentsolver.cpp:271-289 for one cell's moments u_cell (M, S)."""
+    lv = np.ascontiguousarray(nt_level, dtype=np.int32)
+    return lib().orc_ref_indicator(S, d, _p(lv), int(lev), _p(np.ascontiguousarray(u_cell, dtype=np.float64)))
+
+
+def refine(u, d, nt_level, nt_cell, above=0.02, below=0.004):
+    """momentsolver.cpp:104-117 over all cells; returns the new nt_cell."""
+    ncell, M, S = u.shape
+    lv = np.ascontiguousarray(nt_level, dtype=np.int32)
+    nt = np.array(nt_cell, dtype=np.int32, copy=True)
+    lib().orc_refine(S, M, d, len(lv), _p(lv), C.c_double(above), C.c_double(below), C.c_long(ncell),
+                     _p(np.ascontiguousarray(u, dtype=np.float64)), _p(nt))
+    return nt
+
+
+def run_levels(c, p, mesh, phi, phiw, xi, u, lam, nt_level, nt_cell, t_end, max_steps=-1, above=0.02, below=0.004,
+               nthreads=0):
+    """Time loop with refinement levels; u, lam (ncell, M, S) and nt_cell
+    (int32 array, updated) in place."""
+    Q, M = phi.shape
+    d = xi.shape[1]
+    t = C.c_double()
+    steps = C.c_long()
+    res = C.c_double()
+    newton = C.c_long()
+    lv = np.ascontiguousarray(nt_level, dtype=np.int32)
+    assert nt_cell.dtype == np.int32 and nt_cell.flags.c_contiguous
+    rc = lib().orc_run_levels(C.byref(c), C.byref(p), C.byref(mesh.struct), M, Q, d, _p(phi), _p(phiw),
+                              _p(np.ascontiguousarray(xi)), len(lv), _p(lv), C.c_double(above), C.c_double(below),
+                              _p(nt_cell), _p(u), _p(lam), C.c_double(t_end), C.c_long(max_steps), C.byref(t),
+                              C.byref(steps), C.byref(res), C.byref(newton), nthreads)
     return dict(rc=rc, t=t.value, steps=steps.value, residual=res.value, newton_iters=newton.value)
 
 

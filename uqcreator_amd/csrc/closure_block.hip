@@ -194,33 +194,44 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                 V acc[P];
 #pragma unroll
                 for (int pp = 0; pp < P; ++pp) acc[pp] = V{0, 0, 0, 0};
+                // table loads run PF k steps ahead of the products (L2 latency)
+                constexpr int PF = 4;
                 auto load = [&](int ks, double& av, double& bv) {
                     const int k = 4 * ks + kq;
                     const bool kin = k < Q;
                     av = (kin && jin) ? phi[k * M + jr] : 0.0;
                     bv = (kin && iin) ? phiw[k * M + ic] : 0.0;
                 };
-                double av, bv;
-                load(0, av, bv);
-                for (int ks = 0; ks < ks_n; ++ks) {
-                    double av2 = 0.0, bv2 = 0.0;
-                    if (ks + 1 < ks_n) load(ks + 1, av2, bv2);
-                    const int k = 4 * ks + kq;
-                    const double* du = DUk + (k < Q ? k : 0) * P;
+                double av[PF], bv[PF];
 #pragma unroll
-                    for (int l = 0; l < S; ++l)
+                for (int u = 0; u < PF; ++u) load(u, av[u], bv[u]);
+                for (int ks0 = 0; ks0 < ks_n; ks0 += PF) {
+                    double an[PF], bn[PF];
 #pragma unroll
-                        for (int m = 0; m <= l; ++m) {
-                            const int pp = l * (l + 1) / 2 + m;
-                            if (lower && l == m) continue;
-                            if constexpr (PREC == 0)
-                                acc[pp] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, du[pp] * bv, acc[pp], 0, 0, 0);
-                            else
-                                acc[pp] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)av, (float)(du[pp] * bv), acc[pp], 0,
-                                                                                0, 0);
-                        }
-                    av = av2;
-                    bv = bv2;
+                    for (int u = 0; u < PF; ++u) load(ks0 + PF + u, an[u], bn[u]);
+#pragma unroll
+                    for (int u = 0; u < PF; ++u) {
+                        // beyond Q the loaded values are zero, any DU row will do
+                        const int k = 4 * (ks0 + u) + kq;
+                        const double* du = DUk + (k < Q ? k : 0) * P;
+#pragma unroll
+                        for (int l = 0; l < S; ++l)
+#pragma unroll
+                            for (int m = 0; m <= l; ++m) {
+                                const int pp = l * (l + 1) / 2 + m;
+                                if (lower && l == m) continue;
+                                if constexpr (PREC == 0)
+                                    acc[pp] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], du[pp] * bv[u], acc[pp], 0, 0, 0);
+                                else
+                                    acc[pp] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)av[u], (float)(du[pp] * bv[u]),
+                                                                                    acc[pp], 0, 0, 0);
+                            }
+                    }
+#pragma unroll
+                    for (int u = 0; u < PF; ++u) {
+                        av[u] = an[u];
+                        bv[u] = bn[u];
+                    }
                 }
                 if (!iin) continue;
 #pragma unroll

@@ -63,6 +63,36 @@ __global__ void __launch_bounds__(kSweepThreads) reconstruct_kernel(SweepArgs a)
     a.u_out[t] = acc;
 }
 
+// MomentSolver refinement (momentsolver.cpp:104-117) with ComputeRefIndicator
+// (:271-289), one thread per cell.  The indicator of a cell at level l is
+// |u(0, nt-1)| + |u(0, nt-2)| for one uncertain dimension, else the sum of
+// |u(0, i)| over the moments the level adds (i from nTotal(l-1), or from 1 at
+// level 0).  Above refine_above the cell moves up a level, below
+// coarsen_below down.  A cell whose nt_cell is no level's nTotal is left alone.
+__global__ void __launch_bounds__(kSweepThreads) refine_kernel(RefineArgs a) {
+    UQK_SKIP_IF_INACTIVE(a)
+    const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.ncell) return;
+    const int nt = a.nt_cell[j];
+    int lev = -1;
+    for (int l = 0; l < a.nlev; ++l)
+        if (a.nt_level[l] == nt) lev = l;
+    if (lev < 0) return;
+    const double* u = a.u + j * a.S * a.M;
+    double ind = 0.0;
+    if (a.dim_xi == 1) {
+        ind = fabs(u[(nt - 1) * a.S]);
+        if (nt >= 2) ind += fabs(u[(nt - 2) * a.S]);
+    } else {
+        for (int i = lev == 0 ? 1 : a.nt_level[lev - 1]; i < nt; ++i) ind += fabs(u[i * a.S]);
+    }
+    if (ind > a.refine_above && lev < a.nlev - 1)
+        ++lev;
+    else if (ind < a.coarsen_below && lev > 0)
+        --lev;
+    a.nt_cell[j] = a.nt_level[lev];
+}
+
 __device__ __forceinline__ double block_min(double v) {
     __shared__ double red[kSweepThreads / 64];
     v = wave_min(v);
@@ -345,6 +375,13 @@ template <int KIND> static void go_eval_uq(const SweepArgs& a, hipStream_t st) {
 hipError_t launch_eval_uq(int kind, const SweepArgs& a, hipStream_t st) {
     if (a.ncell <= 0) return hipSuccess;
     UQK_DISPATCH(kind, go_eval_uq, a, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_refine(const RefineArgs& a, hipStream_t st) {
+    if (a.ncell <= 0) return hipSuccess;
+    const int blocks = (int)((a.ncell + kSweepThreads - 1) / kSweepThreads);
+    hipLaunchKernelGGL(refine_kernel, dim3(blocks), dim3(kSweepThreads), 0, st, a);
     return hipGetLastError();
 }
 

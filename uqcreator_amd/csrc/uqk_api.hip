@@ -185,11 +185,17 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
 
 // Enqueue one step body (momentsolver.cpp:83-152) on ctx->stream without any
 // host synchronisation: closure, uQ, reconstruction, dt, advance, residual.
-int enqueue_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const int32_t* nt, double* u,
+// With refinement on (ctx->refine.nlev > 1) the levels move between uQ and the
+// reconstruction, as in momentsolver.cpp:99-124: the closure and uQ use the
+// old nTotal, the reconstruction and the advance the new one.
+int enqueue_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, int32_t* nt, double* u,
                  double* lam, double* uq, double* u_scratch, int32_t* status, int32_t* iters, const int* active) {
     const int64_t n = mesh->n_cells;
+    const bool refining = ctx->refine.nlev > 1;
+    if (refining && !nt) return fail(ctx, -1, "refinement levels need ntotal_cell");
     bool fused = false;
-    int rc = launch_closure(ctx, n, nt, u, lam, status, iters, active, uq, u_scratch, &fused);
+    int rc = launch_closure(ctx, n, nt, u, lam, status, iters, active, refining ? nullptr : uq,
+                            refining ? nullptr : u_scratch, &fused);
     if (rc) return rc;
     uqk::SweepArgs a = sweep_args(ctx);
     a.p = problem_params(prob);
@@ -200,6 +206,14 @@ int enqueue_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, co
     a.uq_out = uq;
     a.active = active;
     if (!fused) UQK_HIP(ctx, uqk::launch_eval_uq(ctx->kind, a, ctx->stream));
+    if (refining) {
+        uqk::RefineArgs r = ctx->refine;
+        r.ncell = n;
+        r.u = u;
+        r.nt_cell = nt;
+        r.active = active;
+        UQK_HIP(ctx, uqk::launch_refine(r, ctx->stream));
+    }
     a.uq = uq;
     a.u_out = u_scratch;
     if (!fused) UQK_HIP(ctx, uqk::launch_reconstruct(a, ctx->stream));
@@ -408,6 +422,49 @@ int uqk_eval_uq(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* lam, d
     return 0;
 }
 
+int uqk_set_refinement(uqk_ctx* ctx, int32_t n_levels, const int32_t* ntotal_per_level, double refine_above,
+                       double coarsen_below) {
+    if (!ctx) return -1;
+    if (n_levels <= 1) {
+        ctx->refine = uqk::RefineArgs{};
+        ctx->graph_key.clear();
+        return 0;
+    }
+    if (n_levels > uqk::kMaxLevels || !ntotal_per_level)
+        return fail(ctx, -1, "refinement: 2.." + std::to_string(uqk::kMaxLevels) + " levels and their nTotal");
+    for (int l = 0; l < n_levels; ++l) {
+        const int v = ntotal_per_level[l];
+        if (v < 1 || v > ctx->M || (l > 0 && v <= ntotal_per_level[l - 1]))
+            return fail(ctx, -1, "refinement: nTotal per level must ascend within 1..n_moments");
+    }
+    if (!(coarsen_below <= refine_above)) return fail(ctx, -1, "refinement: coarsen_below > refine_above");
+    uqk::RefineArgs r{};
+    r.S = ctx->S;
+    r.M = ctx->M;
+    r.dim_xi = ctx->d;
+    r.nlev = n_levels;
+    for (int l = 0; l < n_levels; ++l) r.nt_level[l] = ntotal_per_level[l];
+    r.refine_above = refine_above;
+    r.coarsen_below = coarsen_below;
+    ctx->refine = r;
+    ctx->graph_key.clear();  // the captured step changes
+    return 0;
+}
+
+int uqk_refine(uqk_ctx* ctx, int64_t n, const double* u, int32_t* nt) {
+    if (!ctx) return -1;
+    if (ctx->refine.nlev < 2) return fail(ctx, -1, "refinement levels are not set");
+    if (n > 0 && (!u || !nt)) return fail(ctx, -1, "refine: null array");
+    UQK_HIP(ctx, hipSetDevice(ctx->device));
+    uqk::RefineArgs r = ctx->refine;
+    r.ncell = n;
+    r.u = u;
+    r.nt_cell = nt;
+    UQK_HIP(ctx, uqk::launch_refine(r, ctx->stream));
+    UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
 int uqk_reconstruct(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* uq, double* u) {
     if (!ctx) return -1;
     uqk::SweepArgs a = sweep_args(ctx);
@@ -453,7 +510,7 @@ int uqk_advance(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, con
     return 0;
 }
 
-int64_t uqk_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const int32_t* nt, double* u,
+int64_t uqk_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, int32_t* nt, double* u,
                  double* lam, double* uq, double* u_scratch, int32_t* status, int32_t* iters, double* dt_out,
                  double* residual_out) {
     if (!ctx || !prob || !mesh) return -1;
@@ -469,7 +526,7 @@ int64_t uqk_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, co
     return nfail;
 }
 
-int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, const int32_t* nt, double* u,
+int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, int32_t* nt, double* u,
                       double* lam, double* uq, double* u_scratch, int32_t* status, int32_t* iters, double t0,
                       double t_end, int64_t max_steps, int32_t use_graph, double* t_out, int64_t* steps_out,
                       double* residual_out, int64_t* newton_out) {

@@ -42,6 +42,7 @@ struct uqk_ctx {
     int num_cu = 0;
     uqk::LaunchShape shape;   // specialised closure kernel launch shape (autotuned or set)
     uqk_mlp* warm = nullptr;  // learned warm start run before each closure (not owned)
+    uqk::RefineArgs refine{};  // adaptive levels: refine.nlev > 1 turns refinement on in the step
     std::string err;
 };
 

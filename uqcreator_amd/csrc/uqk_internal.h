@@ -107,7 +107,23 @@ struct SweepArgs {
     int table_lds;               // advance: PhiW staged in LDS (set by the launcher)
 };
 
+// Refinement step of the adaptive moment solver: per cell the level whose
+// nTotal equals nt_cell[j], its indicator from the moments u, one level up or
+// down, nt_cell[j] rewritten.
+constexpr int kMaxLevels = 16;
+struct RefineArgs {
+    long ncell;
+    int S, M, dim_xi;
+    int nlev;
+    int nt_level[kMaxLevels];  // nTotal per level, ascending
+    double refine_above, coarsen_below;
+    const double* u;  // moments, (s, i) of cell j at j*S*M + i*S + s
+    int* nt_cell;
+    const int* active;  // device flag: 0 skips the launch
+};
+
 hipError_t launch_eval_uq(int kind, const SweepArgs& a, hipStream_t st);
+hipError_t launch_refine(const RefineArgs& a, hipStream_t st);
 hipError_t launch_reconstruct(const SweepArgs& a, hipStream_t st);
 hipError_t launch_dt(int kind, const SweepArgs& a, hipStream_t st);
 hipError_t launch_advance(int kind, const SweepArgs& a, hipStream_t st);

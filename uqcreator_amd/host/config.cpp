@@ -87,6 +87,9 @@ Settings Settings::from_toml(const Toml& t) {
     s.max_degree = dt == "maxDegree";
     for (double v : numbers(mom->arr[1])) s.degrees.push_back((int)v);
     if (s.degrees.empty()) throw std::runtime_error("[moment_system] 'moments' has no degree");
+    // extension keys: the refinement thresholds the reference hard-codes
+    s.refine_above = t.get_number("moment_system", "refineAbove", 0.02);
+    s.coarsen_below = t.get_number("moment_system", "coarsenBelow", 0.004);
     const TomlValue* qp = t.get("moment_system", "quadPoints");
     if (!qp || qp->kind != TomlValue::ARRAY || qp->arr.size() != 2)
         throw std::runtime_error("[moment_system] 'quadPoints' must be [[\"tensorizedGrid\"], [n]]");

@@ -37,7 +37,8 @@ struct Settings {
     // [moment_system]
     int closure = UQK_BOUNDED_BARRIER;
     bool max_degree = false;
-    std::vector<int> degrees;
+    std::vector<int> degrees;  // one per refinement level
+    double refine_above = 0.02, coarsen_below = 0.004;  // momentsolver.cpp:108-111
     int quad_points = 0;
     int max_iterations = 1000;
     double epsilon = 5e-5;

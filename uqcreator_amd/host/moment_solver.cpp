@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <memory>
 #include <stdexcept>
 #include <thread>
 #include <vector>
@@ -213,6 +214,20 @@ static RunOutput run_config(const std::string& path, const char* mean_var_csv) {
         if (f != 0) throw std::runtime_error("initial closure failed");
     }
     uqk_ctx* ctx = make_ctx(s, b, s.max_iterations, s.safe_line_search);
+    // refinement levels (momentsolver.cpp:39): every cell starts on the finest
+    const int nlev = (int)b.ntotal_per_level.size();
+    std::vector<int32_t> nt_host;
+    if (nlev > 1) {
+        nt_host.assign((size_t)N, b.ntotal_per_level.back());
+        if (uqk_set_refinement(ctx, nlev, b.ntotal_per_level.data(), s.refine_above, s.coarsen_below) != 0) {
+            std::string msg = uqk_last_error(ctx);
+            uqk_destroy(ctx);
+            throw std::runtime_error(msg);
+        }
+    }
+    std::unique_ptr<DeviceBuf> dnt;
+    if (nlev > 1) dnt.reset(new DeviceBuf(nt_host.data(), sizeof(int32_t) * nt_host.size()));
+    int32_t* nt = dnt ? dnt->as<int32_t>() : nullptr;
     auto tic = std::chrono::steady_clock::now();
     int64_t nfail = 0;
     if (s.min_residual > 0.0) {
@@ -220,7 +235,7 @@ static RunOutput run_config(const std::string& path, const char* mean_var_csv) {
         double t = 0.0, res = s.min_residual + 1.0;
         while (t < s.t_end && res > s.min_residual && (s.max_steps < 0 || out.steps < s.max_steps)) {
             double dt = 0.0;
-            nfail = uqk_step(ctx, &prob, &mesh, nullptr, du.as<double>(), dl.as<double>(), duq.as<double>(),
+            nfail = uqk_step(ctx, &prob, &mesh, nt, du.as<double>(), dl.as<double>(), duq.as<double>(),
                              dus.as<double>(), dst.as<int32_t>(), dit.as<int32_t>(), &dt, &res);
             if (nfail) break;
             int64_t a = 0, g = 0;
@@ -232,7 +247,7 @@ static RunOutput run_config(const std::string& path, const char* mean_var_csv) {
         out.t = t;
         out.residual = res;
     } else {
-        nfail = uqk_run_steps(ctx, &prob, &mesh, nullptr, du.as<double>(), dl.as<double>(), duq.as<double>(),
+        nfail = uqk_run_steps(ctx, &prob, &mesh, nt, du.as<double>(), dl.as<double>(), duq.as<double>(),
                               dus.as<double>(), dst.as<int32_t>(), dit.as<int32_t>(), 0.0, s.t_end, s.max_steps, 1,
                               &out.t, &out.steps, &out.residual, &out.newton);
     }
@@ -249,7 +264,7 @@ static RunOutput run_config(const std::string& path, const char* mean_var_csv) {
     }
     if (mean_var_csv) {
         DeviceBuf mv(sizeof(double) * (size_t)N * 2 * S);
-        uqk_mean_var(ctx, N, nullptr, dl.as<double>(), mv.as<double>());
+        uqk_mean_var(ctx, N, nt, dl.as<double>(), mv.as<double>());
         std::vector<double> h((size_t)N * 2 * S);
         uqk_memcpy(h.data(), mv.p, (int64_t)(sizeof(double) * h.size()), 1);
         std::ofstream f(mean_var_csv);

@@ -81,6 +81,8 @@ SIGNATURES = {
                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
                              C.POINTER(C.c_double)]),
     "uqk_mean_var": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uqk_set_refinement": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_double, C.c_double]),
+    "uqk_refine": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "uqk_run_steps": (C.c_int64, [C.c_void_p, C.POINTER(ProblemDesc), C.POINTER(MeshDesc), C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
                                   C.c_int64, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64),
@@ -383,6 +385,17 @@ class GpuClosure:
 
     def mean_var(self, lam, mv, nt_cell=None):
         self._check(lib().uqk_mean_var(self.h, lam.shape[0], _p(nt_cell), _p(lam), _p(mv)), "uqk_mean_var")
+
+    def set_refinement(self, ntotal_per_level, refine_above=0.02, coarsen_below=0.004):
+        """Refinement levels (momentsolver.cpp:104-117); step and run_steps then
+        update nt_cell in place every step.  One level or None turns it off."""
+        lv = np.ascontiguousarray([] if ntotal_per_level is None else ntotal_per_level, dtype=np.int32)
+        self._check(lib().uqk_set_refinement(self.h, len(lv), _p(lv) if len(lv) else None, float(refine_above),
+                                             float(coarsen_below)), "uqk_set_refinement")
+
+    def refine(self, u, nt_cell):
+        """One refinement pass: nt_cell (device int32, ncell) from the moments u."""
+        self._check(lib().uqk_refine(self.h, u.shape[0], _p(u), _p(nt_cell)), "uqk_refine")
 
     def last_closure_ms(self):
         return lib().uqk_last_closure_ms(self.h)

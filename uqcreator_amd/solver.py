@@ -12,7 +12,10 @@ from . import native
 
 
 class GpuMomentSolver:
-    def __init__(self, closure, problem, mesh, nt_cell=None):
+    def __init__(self, closure, problem, mesh, nt_cell=None, levels=False, refine_above=0.02, coarsen_below=0.004):
+        """levels=True turns on the refinement levels of the basis
+        (momentsolver.cpp:104-117): every cell starts on the finest level
+        unless nt_cell says otherwise, and the step moves it between levels."""
         self.cl = closure
         self.prob = problem
         self.mesh = mesh
@@ -25,6 +28,13 @@ class GpuMomentSolver:
         self.u_scratch = native.DeviceArray((n, M, S))
         self.status = native.DeviceArray(n, np.int32)
         self.iters = native.DeviceArray(n, np.int32)
+        if levels:
+            ntl = closure.basis.ntotal_per_level
+            closure.set_refinement(ntl, refine_above, coarsen_below)
+            if nt_cell is None:
+                nt_cell = np.full(n, ntl[-1], np.int32)
+        else:
+            closure.set_refinement(None)
         self.nt = None if nt_cell is None else native.DeviceArray.from_host(np.asarray(nt_cell, np.int32))
         self.t = 0.0
         self.steps = 0
@@ -69,6 +79,10 @@ class GpuMomentSolver:
         self.residual = r["residual"]
         self.newton_steps += r["newton_steps"]
         return self
+
+    def nt_cell(self):
+        """Per-cell nTotal (the refinement levels) as a host array, or None."""
+        return None if self.nt is None else self.nt.download()
 
     def mean_var(self):
         mv = native.DeviceArray((self.mesh.n_cells, 2 * self.cl.S))
