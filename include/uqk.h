@@ -153,6 +153,11 @@ int uqk_reconstruct(uqk_ctx* ctx, int64_t n_cells, const int32_t* ntotal_cell, c
  * update ntotal_cell (then required) in place; n_levels <= 1 turns it off. */
 int uqk_set_refinement(uqk_ctx* ctx, int32_t n_levels, const int32_t* ntotal_per_level, double refine_above,
                        double coarsen_below);
+/* Refinement retardation (Paper1/adaptivity.tex:61-86, not in the reference
+ * code): no cell moves above level max_level (0 .. n_levels-1; a cell above it
+ * comes down to it at its next refinement pass).  uqk_set_refinement resets
+ * the cap to the finest level. */
+int uqk_set_refinement_cap(uqk_ctx* ctx, int32_t max_level);
 /* One refinement pass over n_cells: ntotal_cell (device, in/out) from the
  * moments u (device).  Cells whose ntotal_cell is no level's nTotal are left
  * unchanged.  Synchronous. */

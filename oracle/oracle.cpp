@@ -909,9 +909,10 @@ extern "C" double orc_ref_indicator(int S, int d, const int* nt_level, int lev, 
  * up one level above `above` (0.02 there) unless finest, else down one below
  * `below` (0.004) unless coarsest.  The level of a cell is the one whose
  * nTotal equals nt_cell[j]; nt_cell is rewritten.  Cells whose nt_cell is no
- * level's nTotal are left as they are. */
-extern "C" void orc_refine(int S, int M, int d, int nlev, const int* nt_level, double above, double below, long ncell,
-                           const double* u, int* nt_cell) {
+ * level's nTotal are left as they are.  No cell ends above max_level, the cap
+ * of refinement retardation (Paper1/adaptivity.tex:61-86; nlev-1 = none). */
+extern "C" void orc_refine(int S, int M, int d, int nlev, const int* nt_level, double above, double below,
+                           int max_level, long ncell, const double* u, int* nt_cell) {
     for (long j = 0; j < ncell; ++j) {
         int lev = -1;
         for (int l = 0; l < nlev; ++l)
@@ -922,6 +923,7 @@ extern "C" void orc_refine(int S, int M, int d, int nlev, const int* nt_level, d
             ++lev;
         else if (ind < below && lev > 0)
             --lev;
+        if (lev > max_level) lev = max_level;
         nt_cell[j] = nt_level[lev];
     }
 }
@@ -933,7 +935,8 @@ extern "C" void orc_refine(int S, int M, int d, int nlev, const int* nt_level, d
  * reconstruction and Advance the new one. */
 static int run_impl(const orc_closure* c, const orc_problem* p, const orc_mesh* mesh, int M, int Q, int d,
                     const double* phi, const double* phiw, const double* xi, int* nt_cell, int nlev,
-                    const int* nt_level, double above, double below, double* u, double* lam, double t_end,
+                    const int* nt_level, double above, double below, int max_level, double* u, double* lam,
+                    double t_end,
                     long max_steps, double* t_out, long* steps_out, double* residual_out, long* newton_iters_out,
                     int nthreads) {
     const int S = c->S;
@@ -952,7 +955,7 @@ static int run_impl(const orc_closure* c, const orc_problem* p, const orc_mesh* 
         }
         for (long j = 0; j < N; ++j) uold0[j] = u[(size_t)j * S * M];
         orc_eval_uq(c, M, Q, phi, N, nt_cell, lam, uq.data(), nthreads);
-        if (nlev > 1) orc_refine(S, M, d, nlev, nt_level, above, below, N, u, nt_cell);
+        if (nlev > 1) orc_refine(S, M, d, nlev, nt_level, above, below, max_level, N, u, nt_cell);
         orc_reconstruct(S, M, Q, phiw, N, nt_cell, uq.data(), u, nthreads);
         double dt = 1e10;
         for (long j = 0; j < N; ++j) dt = std::min(dt, orc_compute_dt(p, S, Q, uq.data() + (size_t)j * S * Q, mesh->dx_param[j]));
@@ -974,17 +977,18 @@ extern "C" int orc_run(const orc_closure* c, const orc_problem* p, const orc_mes
                        const double* phi, const double* phiw, const double* xi, const int* nt_cell, double* u,
                        double* lam, double t_end, long max_steps, double* t_out, long* steps_out,
                        double* residual_out, long* newton_iters_out, int nthreads) {
-    return run_impl(c, p, mesh, M, Q, d, phi, phiw, xi, const_cast<int*>(nt_cell), 0, nullptr, 0.0, 0.0, u, lam,
+    return run_impl(c, p, mesh, M, Q, d, phi, phiw, xi, const_cast<int*>(nt_cell), 0, nullptr, 0.0, 0.0, 0, u, lam,
                     t_end, max_steps, t_out, steps_out, residual_out, newton_iters_out, nthreads);
 }
 
-/* The same loop with refinement levels; nt_cell (required) is updated. */
+/* The same loop with refinement levels (cap max_level); nt_cell (required)
+ * is updated. */
 extern "C" int orc_run_levels(const orc_closure* c, const orc_problem* p, const orc_mesh* mesh, int M, int Q, int d,
                               const double* phi, const double* phiw, const double* xi, int nlev, const int* nt_level,
-                              double above, double below, int* nt_cell, double* u, double* lam, double t_end,
-                              long max_steps, double* t_out, long* steps_out, double* residual_out,
+                              double above, double below, int max_level, int* nt_cell, double* u, double* lam,
+                              double t_end, long max_steps, double* t_out, long* steps_out, double* residual_out,
                               long* newton_iters_out, int nthreads) {
-    return run_impl(c, p, mesh, M, Q, d, phi, phiw, xi, nt_cell, nlev, nt_level, above, below, u, lam, t_end,
+    return run_impl(c, p, mesh, M, Q, d, phi, phiw, xi, nt_cell, nlev, nt_level, above, below, max_level, u, lam, t_end,
                     max_steps, t_out, steps_out, residual_out, newton_iters_out, nthreads);
 }
 

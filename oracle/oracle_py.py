@@ -281,18 +281,19 @@ def ref_indicator(S, d, nt_level, lev, u_cell):
     return lib().orc_ref_indicator(S, d, _p(lv), int(lev), _p(np.ascontiguousarray(u_cell, dtype=np.float64)))
 
 
-def refine(u, d, nt_level, nt_cell, above=0.02, below=0.004):
+def refine(u, d, nt_level, nt_cell, above=0.02, below=0.004, max_level=None):
     """momentsolver.cpp:104-117 over all cells; returns the new nt_cell."""
     ncell, M, S = u.shape
     lv = np.ascontiguousarray(nt_level, dtype=np.int32)
     nt = np.array(nt_cell, dtype=np.int32, copy=True)
-    lib().orc_refine(S, M, d, len(lv), _p(lv), C.c_double(above), C.c_double(below), C.c_long(ncell),
+    cap = len(lv) - 1 if max_level is None else int(max_level)
+    lib().orc_refine(S, M, d, len(lv), _p(lv), C.c_double(above), C.c_double(below), cap, C.c_long(ncell),
                      _p(np.ascontiguousarray(u, dtype=np.float64)), _p(nt))
     return nt
 
 
 def run_levels(c, p, mesh, phi, phiw, xi, u, lam, nt_level, nt_cell, t_end, max_steps=-1, above=0.02, below=0.004,
-               nthreads=0):
+               max_level=None, nthreads=0):
     """Time loop with refinement levels; u, lam (ncell, M, S) and nt_cell
     (int32 array, updated) in place."""
     Q, M = phi.shape
@@ -302,10 +303,11 @@ def run_levels(c, p, mesh, phi, phiw, xi, u, lam, nt_level, nt_cell, t_end, max_
     res = C.c_double()
     newton = C.c_long()
     lv = np.ascontiguousarray(nt_level, dtype=np.int32)
+    cap = len(lv) - 1 if max_level is None else int(max_level)
     assert nt_cell.dtype == np.int32 and nt_cell.flags.c_contiguous
     rc = lib().orc_run_levels(C.byref(c), C.byref(p), C.byref(mesh.struct), M, Q, d, _p(phi), _p(phiw),
                               _p(np.ascontiguousarray(xi)), len(lv), _p(lv), C.c_double(above), C.c_double(below),
-                              _p(nt_cell), _p(u), _p(lam), C.c_double(t_end), C.c_long(max_steps), C.byref(t),
+                              cap, _p(nt_cell), _p(u), _p(lam), C.c_double(t_end), C.c_long(max_steps), C.byref(t),
                               C.byref(steps), C.byref(res), C.byref(newton), nthreads)
     return dict(rc=rc, t=t.value, steps=steps.value, residual=res.value, newton_iters=newton.value)
 

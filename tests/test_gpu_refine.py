@@ -38,6 +38,13 @@ def test_refine_pass_matches_oracle(oracle, kind, d, degrees):
     assert np.array_equal(got, want)
     moved = (want != nt).mean()
     assert 0.2 < moved < 0.9, moved
+    # with a cap on the level (refinement retardation)
+    cl.set_refinement_cap(1)
+    dnt.upload(nt)
+    cl.refine(du, dnt)
+    want1 = oracle.refine(u, d, ntl, nt, max_level=1)
+    assert np.array_equal(dnt.download(), want1)
+    assert want1.max() == ntl[1]
     cl.set_refinement(None)
 
 
@@ -129,4 +136,53 @@ def test_adaptive_input_file(oracle, tmp_path):
     err = _rel_l2(mv, mv_o, m.area)
     print("report", rep, "levels", np.unique(nto, return_counts=True), "rel L2", err, flush=True)
     assert len(np.unique(nto)) >= 2
+    assert err.max() < 1e-8
+
+
+def test_adaptive_one_shot_retardation_input(oracle, tmp_path):
+    """Adaptive One-Shot IPM with refinement retardation through
+    uqk_run_config (steady branch): duals converged once at the start, one
+    Newton step per cell and time step after that, cells start on level 0
+    under a cap that rises to level 1 once the residual falls below 1.58e-5
+    (after the first step) and would rise to 2 below 1e-9 (never here).  The
+    oracle runs the same schedule step by step; without the cap some cells
+    would reach level 2 within these steps."""
+    text = open(os.path.join(INPUT, "euler2d_naca_adaptive.toml")).read()
+    for a, bb in [("ni = 512", "ni = 64"), ("nj = 128", "nj = 16"), ("maxSteps = 2000", "maxSteps = 20\nresidual = 1e-12\n"
+                  "retardationResiduals = [1.58e-5, 1e-9]"), ("maxIterations = 1000", "maxIterations = 1"),
+                  ("refineAbove = 2e-3", "refineAbove = 1e-4"), ("coarsenBelow = 4e-4", "coarsenBelow = 2e-5")]:
+        assert a in text
+        text = text.replace(a, bb)
+    path = tmp_path / "naca_oneshot.toml"
+    path.write_text(text)
+    csv = str(tmp_path / "mv.csv")
+    print("case: adaptive one-shot with retardation", flush=True)
+    rep = native.run_config(str(path), csv)
+    inf, u, lam, center = native.config_setup(str(path))
+    b = native.Basis(2, [1, 2, 3], 5)
+    m = meshes.naca_ogrid(64, 16)
+    ntl = b.ntotal_per_level
+    c0 = _oracle_closure(oracle, native.EULER2D, eps=1e-6, max_iter=10000, safe=1)
+    st, _ = oracle.solve_closure(c0, b.phi, b.phiw, u, lam)
+    assert not st.any()
+    c = _oracle_closure(oracle, native.EULER2D, eps=1e-6, max_iter=1)
+    oprob = oracle.problem(oracle.P_EULER2D, flux=oracle.FLUX_LF, cfl=0.4)
+    om = _oracle_mesh(oracle, m)
+    nt = np.full(m.n_cells, ntl[0], np.int32)
+    tol, rl, caps = [1.58e-5, 1e-9], 0, []
+    for k in range(20):
+        caps.append(min(rl, len(ntl) - 1))
+        r = oracle.run_levels(c, oprob, om, b.phi, b.phiw, b.xi, u, lam, ntl, nt, 1e3, 1, above=1e-4, below=2e-5,
+                              max_level=caps[-1])
+        assert r["rc"] == 0
+        while rl < len(tol) and r["residual"] < tol[rl]:
+            rl += 1
+    assert caps[0] == 0 and caps[1:] == [1] * 19
+    assert rep["steps"] == 20
+    assert rep["residual"] == pytest.approx(r["residual"], rel=1e-9)
+    mv = np.loadtxt(csv, delimiter=",", ndmin=2)
+    mv_o = oracle.mean_var(c, b.phi, b.phiw, lam, nt_cell=nt)
+    err = _rel_l2(mv, mv_o, m.area)
+    print("report", rep, "levels", np.unique(nt, return_counts=True), "rel L2", err, flush=True)
+    assert set(np.unique(nt).tolist()) == {ntl[0], ntl[1]}
     assert err.max() < 1e-8

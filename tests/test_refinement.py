@@ -60,6 +60,22 @@ def test_refine_rule(oracle):
     assert out[-1] == 4
 
 
+def test_refine_cap(oracle):
+    """refinement retardation: no cell ends above the cap; one above it comes
+    down to it whatever its indicator"""
+    d, degrees = 2, [1, 2, 3]
+    idx, ntl = native.index_set(d, degrees)
+    M, S = len(idx), 4
+    cases = [(0, 0.03, 1), (1, 0.03, 1), (2, 0.03, 1), (2, 0.01, 1), (2, 0.001, 1), (1, 0.001, 0), (0, 0.01, 0)]
+    u = np.zeros((len(cases), M, S))
+    nt = np.zeros(len(cases), np.int32)
+    for j, (lev, ind, _) in enumerate(cases):
+        u[j, ntl[lev - 1] if lev > 0 else 1, 0] = ind
+        nt[j] = ntl[lev]
+    out = oracle.refine(u, d, ntl, nt, max_level=1)
+    assert out.tolist() == [ntl[w] for _, _, w in cases]
+
+
 def _burgers_levels(n_cells=200):
     b = native.Basis(1, [2, 3, 4, 5], 12)
     m = meshes.mesh1d(n_cells, 0.0, 3.0)

@@ -68,7 +68,8 @@ __global__ void __launch_bounds__(kSweepThreads) reconstruct_kernel(SweepArgs a)
 // |u(0, nt-1)| + |u(0, nt-2)| for one uncertain dimension, else the sum of
 // |u(0, i)| over the moments the level adds (i from nTotal(l-1), or from 1 at
 // level 0).  Above refine_above the cell moves up a level, below
-// coarsen_below down.  A cell whose nt_cell is no level's nTotal is left alone.
+// coarsen_below down, and never above max_level (the cap of refinement
+// retardation).  A cell whose nt_cell is no level's nTotal is left alone.
 __global__ void __launch_bounds__(kSweepThreads) refine_kernel(RefineArgs a) {
     UQK_SKIP_IF_INACTIVE(a)
     const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -90,6 +91,7 @@ __global__ void __launch_bounds__(kSweepThreads) refine_kernel(RefineArgs a) {
         ++lev;
     else if (ind < a.coarsen_below && lev > 0)
         --lev;
+    if (lev > a.max_level) lev = a.max_level;
     a.nt_cell[j] = a.nt_level[lev];
 }
 

@@ -443,11 +443,23 @@ int uqk_set_refinement(uqk_ctx* ctx, int32_t n_levels, const int32_t* ntotal_per
     r.M = ctx->M;
     r.dim_xi = ctx->d;
     r.nlev = n_levels;
+    r.max_level = n_levels - 1;
     for (int l = 0; l < n_levels; ++l) r.nt_level[l] = ntotal_per_level[l];
     r.refine_above = refine_above;
     r.coarsen_below = coarsen_below;
     ctx->refine = r;
     ctx->graph_key.clear();  // the captured step changes
+    return 0;
+}
+
+int uqk_set_refinement_cap(uqk_ctx* ctx, int32_t max_level) {
+    if (!ctx) return -1;
+    if (ctx->refine.nlev < 2) return fail(ctx, -1, "refinement levels are not set");
+    if (max_level < 0 || max_level >= ctx->refine.nlev) return fail(ctx, -1, "refinement cap outside the levels");
+    if (ctx->refine.max_level != max_level) {
+        ctx->refine.max_level = max_level;
+        ctx->graph_key.clear();  // the captured step changes
+    }
     return 0;
 }
 

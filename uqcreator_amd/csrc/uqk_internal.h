@@ -115,6 +115,7 @@ struct RefineArgs {
     long ncell;
     int S, M, dim_xi;
     int nlev;
+    int max_level;             // cap on the level (refinement retardation), nlev - 1 without
     int nt_level[kMaxLevels];  // nTotal per level, ascending
     double refine_above, coarsen_below;
     const double* u;  // moments, (s, i) of cell j at j*S*M + i*S + s

@@ -70,6 +70,7 @@ Settings Settings::from_toml(const Toml& t) {
     if (!t.has("problem", "tEnd")) throw std::runtime_error("[problem] 'tEnd' not set");
     s.t_end = t.get_number("problem", "tEnd", 0.0);
     s.min_residual = t.get_number("problem", "residual", -1.0);
+    if (const TomlValue* rt = t.get("problem", "retardationResiduals")) s.retardation = numbers(*rt);
     s.gamma = t.get_number("problem", "gamma", 1.4);
     const std::string flux = t.get_string("problem", "flux", "reference");
     s.flux = (flux == "laxFriedrichs" || flux == "rusanov") ? UQK_FLUX_LAX_FRIEDRICHS : UQK_FLUX_REFERENCE;

@@ -29,6 +29,9 @@ struct Settings {
     // [problem]
     std::vector<double> sigma;
     double cfl = 0.5, t_end = 0.0, min_residual = -1.0, gamma = 1.4;
+    // refinement retardation (steady runs): the level cap rises by one each
+    // time the residual falls below the next of these, descending
+    std::vector<double> retardation;
     int flux = UQK_FLUX_REFERENCE;
     long max_steps = -1;
     double friction = 0.0;

@@ -213,17 +213,16 @@ static RunOutput run_config(const std::string& path, const char* mean_var_csv) {
         uqk_destroy(c0);
         if (f != 0) throw std::runtime_error("initial closure failed");
     }
-    uqk_ctx* ctx = make_ctx(s, b, s.max_iterations, s.safe_line_search);
+    std::unique_ptr<uqk_ctx, int (*)(uqk_ctx*)> owner(make_ctx(s, b, s.max_iterations, s.safe_line_search),
+                                                      uqk_destroy);
+    uqk_ctx* ctx = owner.get();
     // refinement levels (momentsolver.cpp:39): every cell starts on the finest
     const int nlev = (int)b.ntotal_per_level.size();
     std::vector<int32_t> nt_host;
     if (nlev > 1) {
         nt_host.assign((size_t)N, b.ntotal_per_level.back());
-        if (uqk_set_refinement(ctx, nlev, b.ntotal_per_level.data(), s.refine_above, s.coarsen_below) != 0) {
-            std::string msg = uqk_last_error(ctx);
-            uqk_destroy(ctx);
-            throw std::runtime_error(msg);
-        }
+        if (uqk_set_refinement(ctx, nlev, b.ntotal_per_level.data(), s.refine_above, s.coarsen_below) != 0)
+            throw std::runtime_error(uqk_last_error(ctx));
     }
     std::unique_ptr<DeviceBuf> dnt;
     if (nlev > 1) dnt.reset(new DeviceBuf(nt_host.data(), sizeof(int32_t) * nt_host.size()));
@@ -231,9 +230,25 @@ static RunOutput run_config(const std::string& path, const char* mean_var_csv) {
     auto tic = std::chrono::steady_clock::now();
     int64_t nfail = 0;
     if (s.min_residual > 0.0) {
-        // steady runs stop on the residual (momentsolver.cpp:79): step by step
+        // steady runs stop on the residual (momentsolver.cpp:79): step by step.
+        // With refinement retardation (Paper1/adaptivity.tex:61-86) the cells
+        // start on level 0 under a level cap that rises by one whenever the
+        // residual falls below the next tolerance; the run ends on the
+        // residual only once the cap has reached the finest level.
+        const bool retard = nlev > 1 && !s.retardation.empty();
+        size_t rl = 0;
+        auto set_cap = [&](int cap) {
+            if (uqk_set_refinement_cap(ctx, cap) != 0) throw std::runtime_error(uqk_last_error(ctx));
+        };
+        if (retard) {
+            set_cap(0);
+            std::vector<int32_t> nt0((size_t)N, b.ntotal_per_level[0]);
+            if (uqk_memcpy(nt, nt0.data(), (int64_t)(sizeof(int32_t) * nt0.size()), 0) != 0)
+                throw std::runtime_error("host to device copy failed");
+        }
         double t = 0.0, res = s.min_residual + 1.0;
-        while (t < s.t_end && res > s.min_residual && (s.max_steps < 0 || out.steps < s.max_steps)) {
+        while (t < s.t_end && (res > s.min_residual || (retard && rl < s.retardation.size())) &&
+               (s.max_steps < 0 || out.steps < s.max_steps)) {
             double dt = 0.0;
             nfail = uqk_step(ctx, &prob, &mesh, nt, du.as<double>(), dl.as<double>(), duq.as<double>(),
                              dus.as<double>(), dst.as<int32_t>(), dit.as<int32_t>(), &dt, &res);
@@ -243,6 +258,11 @@ static RunOutput run_config(const std::string& path, const char* mean_var_csv) {
             out.newton += a;
             t += dt;
             ++out.steps;
+            if (retard) {
+                const size_t rl0 = rl;
+                while (rl < s.retardation.size() && res < s.retardation[rl]) ++rl;
+                if (rl != rl0) set_cap((int)std::min<size_t>(rl, (size_t)nlev - 1));
+            }
         }
         out.t = t;
         out.residual = res;
@@ -253,15 +273,8 @@ static RunOutput run_config(const std::string& path, const char* mean_var_csv) {
     }
     auto toc = std::chrono::steady_clock::now();
     out.runtime_s = std::chrono::duration<double>(toc - tic).count();
-    if (nfail < 0) {
-        std::string msg = uqk_last_error(ctx);
-        uqk_destroy(ctx);
-        throw std::runtime_error(msg);
-    }
-    if (nfail > 0) {
-        uqk_destroy(ctx);
-        throw std::runtime_error("[closure] Newton failed in " + std::to_string(nfail) + " cells");
-    }
+    if (nfail < 0) throw std::runtime_error(uqk_last_error(ctx));
+    if (nfail > 0) throw std::runtime_error("[closure] Newton failed in " + std::to_string(nfail) + " cells");
     if (mean_var_csv) {
         DeviceBuf mv(sizeof(double) * (size_t)N * 2 * S);
         uqk_mean_var(ctx, N, nt, dl.as<double>(), mv.as<double>());
@@ -274,7 +287,6 @@ static RunOutput run_config(const std::string& path, const char* mean_var_csv) {
             f << "\n";
         }
     }
-    uqk_destroy(ctx);
     return out;
 }
 

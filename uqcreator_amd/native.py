@@ -83,6 +83,7 @@ SIGNATURES = {
     "uqk_mean_var": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
     "uqk_set_refinement": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_double, C.c_double]),
     "uqk_refine": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "uqk_set_refinement_cap": (C.c_int, [C.c_void_p, C.c_int32]),
     "uqk_run_steps": (C.c_int64, [C.c_void_p, C.POINTER(ProblemDesc), C.POINTER(MeshDesc), C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
                                   C.c_int64, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64),
@@ -392,6 +393,10 @@ class GpuClosure:
         lv = np.ascontiguousarray([] if ntotal_per_level is None else ntotal_per_level, dtype=np.int32)
         self._check(lib().uqk_set_refinement(self.h, len(lv), _p(lv) if len(lv) else None, float(refine_above),
                                              float(coarsen_below)), "uqk_set_refinement")
+
+    def set_refinement_cap(self, max_level):
+        """Refinement retardation: no cell above level max_level."""
+        self._check(lib().uqk_set_refinement_cap(self.h, int(max_level)), "uqk_set_refinement_cap")
 
     def refine(self, u, nt_cell):
         """One refinement pass: nt_cell (device int32, ncell) from the moments u."""
