@@ -310,6 +310,18 @@ typedef struct {
  * Returns 0, or -1 with the message in err. */
 int uqk_run_config(const char* toml_path, const char* mean_var_csv, uqk_run_report* report, char* err,
                    int32_t errlen);
+/* SU2 mesh reader (Mesh2D::LoadSU2MeshFromFile, mesh2d.cpp:68-255, with the
+ * geometry of triangle.cpp / quadrangle.cpp / cell.cpp).  bc_spec types the
+ * markers: "tag=type;tag=type" with type dirichlet | noslip | swwall |
+ * neumann | periodic (the last two act as untyped boundaries, as in
+ * expliciteuler.cpp).  Triangle or quadrilateral meshes (not mixed), element
+ * ids 0..NELEM-1 in file order.  Returns the number of cells (>= 0) and the
+ * edges per cell in *n_edges; the uqk_mesh arrays (n_cells x n_edges, x 2 for
+ * normals) and center (n_cells x 2) are filled where not NULL, so a first call
+ * with NULL arrays sizes them.  -1 with the message in err on failure. */
+int64_t uqk_su2_mesh(const char* path, const char* bc_spec, int32_t* n_edges, int32_t* neighbor,
+                     double* unit_normal, double* normal, double* area, double* dx_param, int32_t* bc,
+                     double* boundary_normal, double* center, char* err, int32_t errlen);
 /* Host-only part of uqk_run_config: parses the file and builds the mesh and
  * the initial moments without touching a GPU.  info receives (closure, S, M,
  * Q, d, edges per cell, mesh dim, precision); u and lambda (n_cells x M x S)

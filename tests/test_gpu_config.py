@@ -81,3 +81,25 @@ def test_shallow_water_input_short_run(oracle, tmp_path):
     print("report", rep, "rel L2 E/Var", err, flush=True)
     assert rep["steps"] == 4
     assert err.max() < 1e-8
+
+
+def test_euler2d_su2_triangles_input(oracle, tmp_path):
+    """Euler 2D on an SU2 triangle mesh (the 64 x 16 O-grid nodes split into
+    2048 triangles, profile 'noslip', far field 'dirichlet') through an input
+    file with format = "SU2", 10 steps, against the oracle's loop on the mesh
+    the same reader returns."""
+    meshes.naca_ogrid_su2(str(tmp_path / "naca_tri.su2"), 64, 16)
+    text = open(os.path.join(INPUT, "euler2d_naca.toml")).read()
+    for a, b in [('type = "nacaOGrid"', 'format = "SU2"\nSU2MeshFile = "naca_tri.su2"\n'
+                  'SU2BC = [["farfield", "dirichlet"], ["airfoil", "noslip"]]'), ("maxSteps = 2000", "maxSteps = 10")]:
+        assert a in text
+        text = text.replace(a, b)
+    path = tmp_path / "naca_su2.toml"
+    path.write_text(text)
+    m = meshes.su2(str(tmp_path / "naca_tri.su2"), {"farfield": "dirichlet", "airfoil": "noslip"})
+    rep, err = _run(oracle, str(path), tmp_path, native.Basis(2, [3], 5), m, native.EULER2D,
+                    dict(kind=native.EULER2D_PROBLEM, flux=native.FLUX_LAX_FRIEDRICHS, cfl=0.4), dict(eps=1e-6),
+                    1e3, 10)
+    print("report", rep, "rel L2 E/Var", err, flush=True)
+    assert rep["steps"] == 10 and rep["n_cells"] == 2048
+    assert err.max() < 1e-8

@@ -31,6 +31,13 @@ int closure_kind(const std::string& s) {
 
 }  // namespace
 
+Settings Settings::from_file(const std::string& path) {
+    Settings s = from_toml(Toml::parse_file(path));
+    const size_t slash = path.find_last_of('/');
+    s.input_dir = slash == std::string::npos ? "." : path.substr(0, slash);
+    return s;
+}
+
 Settings Settings::from_toml(const Toml& t) {
     Settings s;
     s.problem = t.get_string("general", "problem", "");
@@ -39,6 +46,18 @@ Settings Settings::from_toml(const Toml& t) {
     s.output_dir = t.get_string("general", "outputDir", ".");
     s.dimension = (int)t.get_number("mesh", "dimension", s.problem == "Burgers1D" ? 1 : 2);
     s.mesh_type = t.get_string("mesh", "type", s.dimension == 1 ? "line" : "nacaOGrid");
+    if (t.get_string("mesh", "format", "") == "SU2") {  // mesh2d.cpp:3-61
+        s.mesh_type = "SU2";
+        s.su2_file = t.get_string("mesh", "SU2MeshFile", "");
+        if (s.su2_file.empty()) throw std::runtime_error("[mesh] 'SU2MeshFile' not set");
+        const TomlValue* bcv = t.get("mesh", "SU2BC");
+        if (!bcv || bcv->kind != TomlValue::ARRAY) throw std::runtime_error("[mesh] 'SU2BC' not set");
+        for (const TomlValue& e : bcv->arr) {
+            if (e.kind != TomlValue::ARRAY || e.arr.size() != 2)
+                throw std::runtime_error("[mesh] 'SU2BC' entries are [tag, type]");
+            s.su2_bc.push_back({e.arr[0].str, e.arr[1].str});
+        }
+    }
     s.n_cells_1d = (long)t.get_number("mesh", "numberOfCells", 0);
     s.a = t.get_number("mesh", "a", 0.0);
     s.b = t.get_number("mesh", "b", 0.0);
@@ -304,11 +323,15 @@ Mesh Mesh::from_settings(const Settings& s) {
         return line(s.n_cells_1d, s.a, s.b);
     }
     if (s.mesh_type == "nacaOGrid") return naca_ogrid(s.ni, s.nj, s.radius, s.first_height);
+    if (s.mesh_type == "SU2") {
+        const std::string p = (!s.su2_file.empty() && s.su2_file[0] == '/') ? s.su2_file : s.input_dir + "/" + s.su2_file;
+        return su2(p, s.su2_bc);
+    }
     if (s.mesh_type == "quadGrid") {
         if (s.nx <= 0 || s.ny <= 0) throw std::runtime_error("[mesh] 'nx', 'ny' not set");
         return quad_grid(s.nx, s.ny, s.x_min, s.x_max, s.y_min, s.y_max, s.boundary);
     }
-    throw std::runtime_error("[mesh] 'type' must be line, nacaOGrid or quadGrid (SU2 files are not read here)");
+    throw std::runtime_error("[mesh] 'type' must be line, nacaOGrid or quadGrid, or format = \"SU2\"");
 }
 
 void initial_state(const Settings& s, const double* x, const double* xi, double* out) {

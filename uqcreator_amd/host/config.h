@@ -18,7 +18,10 @@ struct Settings {
     std::string output_dir = ".";
     // [mesh]
     int dimension = 1;
-    std::string mesh_type;        // line | nacaOGrid | quadGrid
+    std::string mesh_type;        // line | nacaOGrid | quadGrid | SU2 (format = "SU2")
+    std::string su2_file;         // SU2MeshFile, relative to the input file's directory
+    std::vector<std::pair<std::string, std::string>> su2_bc;  // SU2BC: marker tag -> type
+    std::string input_dir = ".";
     long n_cells_1d = 0;
     double a = 0.0, b = 0.0;
     int ni = 512, nj = 128;
@@ -54,6 +57,8 @@ struct Settings {
     int dim_xi = 0, n_states = 0;
 
     static Settings from_toml(const Toml& t);
+    // from_toml of the file, with input_dir its directory (settings.cpp:30)
+    static Settings from_file(const std::string& path);
 };
 
 struct Mesh {
@@ -67,6 +72,8 @@ struct Mesh {
                      bool periodic_i, int bc_j0, int bc_j1, int bc_i0, int bc_i1);
     static Mesh quad_grid(int nx, int ny, double x0, double x1, double y0, double y1, int bc);
     static Mesh naca_ogrid(int ni, int nj, double radius, double first_height);
+    // SU2 file (mesh2d.cpp:68-255): triangles or quadrilaterals, markers typed by bcs
+    static Mesh su2(const std::string& path, const std::vector<std::pair<std::string, std::string>>& bcs);
     static Mesh from_settings(const Settings& s);
 };
 

@@ -174,7 +174,7 @@ struct RunOutput {
 };
 
 static RunOutput run_config(const std::string& path, const char* mean_var_csv) {
-    const Settings s = Settings::from_toml(Toml::parse_file(path));
+    const Settings s = Settings::from_file(path);
     const Basis b = make_basis(s);
     const Mesh m = Mesh::from_settings(s);
     const int S = s.n_states, M = b.M, Q = b.Q;
@@ -323,7 +323,7 @@ int uqk_run_config(const char* toml_path, const char* mean_var_csv, uqk_run_repo
 int64_t uqk_config_setup(const char* toml_path, int32_t* info, double* u, double* lambda, double* center, char* err,
                          int32_t errlen) {
     try {
-        const uqk_host::Settings s = uqk_host::Settings::from_toml(uqk_host::Toml::parse_file(toml_path));
+        const uqk_host::Settings s = uqk_host::Settings::from_file(toml_path);
         const uqk_host::Basis b = uqk_host::make_basis(s);
         const uqk_host::Mesh m = uqk_host::Mesh::from_settings(s);
         if (info) {

@@ -31,6 +31,80 @@ class MeshArrays:
         return self.nbr.shape[0]
 
 
+def su2(path, bcs):
+    """An SU2 mesh file (mesh2d.cpp:68-255) read by the library's C++ reader
+    (uqk_su2_mesh); bcs maps marker tags to dirichlet | noslip | swwall |
+    neumann | periodic."""
+    import ctypes as C
+
+    spec = ";".join("%s=%s" % kv for kv in dict(bcs).items()).encode()
+    err = C.create_string_buffer(512)
+    E = C.c_int32()
+    lib = native.lib()
+    n = lib.uqk_su2_mesh(path.encode(), spec, C.byref(E), *([None] * 8), err, 512)
+    if n < 0:
+        raise RuntimeError(err.value.decode())
+    E = E.value
+    nbr = np.zeros((n, E), np.int32)
+    un = np.zeros((n, E, 2))
+    sn = np.zeros((n, E, 2))
+    area = np.zeros(n)
+    dx = np.zeros(n)
+    bc = np.zeros(n, np.int32)
+    bn = np.zeros((n, 2))
+    center = np.zeros((n, 2))
+    p = native._p
+    n2 = lib.uqk_su2_mesh(path.encode(), spec, None, p(nbr), p(un), p(sn), p(area), p(dx), p(bc), p(bn), p(center),
+                          err, 512)
+    if n2 != n:
+        raise RuntimeError(err.value.decode())
+    return MeshArrays(nbr, un, sn, area, dx, bc, bn, center)
+
+
+def write_su2(path, nodes, cells, markers):
+    """Writes an SU2 mesh: nodes (P, 2), cells (N, 3) triangles or (N, 4)
+    quadrilaterals of node indices, markers {tag: (K, 2) node pairs}."""
+    cells = np.asarray(cells)
+    etype = {3: 5, 4: 9}[cells.shape[1]]
+    with open(path, "w") as f:
+        f.write("NDIME= 2\nNELEM= %d\n" % len(cells))
+        for i, c in enumerate(cells):
+            f.write("%d %s %d\n" % (etype, " ".join(str(int(v)) for v in c), i))
+        f.write("NPOIN= %d\n" % len(nodes))
+        for i, (x, y) in enumerate(nodes):
+            f.write("%.17g %.17g %d\n" % (x, y, i))
+        f.write("NMARK= %d\n" % len(markers))
+        for tag, pairs in markers.items():
+            f.write("MARKER_TAG= %s\nMARKER_ELEMS= %d\n" % (tag, len(pairs)))
+            for a, b in pairs:
+                f.write("3 %d %d\n" % (a, b))
+
+
+def grid_su2(path, nx, ny, x0, x1, y0, y1, triangles=True, tags=("left", "right", "bottom", "top")):
+    """A uniform nx x ny grid of [x0,x1] x [y0,y1] written as SU2 (each
+    square split into two triangles, or kept as a quadrilateral), with one
+    marker per side.  Returns (nodes, cells)."""
+    xs = np.linspace(x0, x1, nx + 1)
+    ys = np.linspace(y0, y1, ny + 1)
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    nodes = np.stack([X.ravel(), Y.ravel()], 1)
+    nid = lambda i, j: i * (ny + 1) + j
+    cells = []
+    for j in range(ny):
+        for i in range(nx):
+            a, b, c, d = nid(i, j), nid(i + 1, j), nid(i + 1, j + 1), nid(i, j + 1)
+            if triangles:
+                cells += [(a, b, c), (a, c, d)] if (i + j) % 2 == 0 else [(a, b, d), (b, c, d)]
+            else:
+                cells.append((a, b, c, d))
+    markers = {tags[0]: [(nid(0, j), nid(0, j + 1)) for j in range(ny)],
+               tags[1]: [(nid(nx, j), nid(nx, j + 1)) for j in range(ny)],
+               tags[2]: [(nid(i, 0), nid(i + 1, 0)) for i in range(nx)],
+               tags[3]: [(nid(i, ny), nid(i + 1, ny)) for i in range(nx)]}
+    write_su2(path, nodes, cells, markers)
+    return nodes, np.asarray(cells)
+
+
 def mesh1d(n_cells, a, b):
     """Mesh1D (mesh1d.cpp:5-67, line.cpp:3-44): nodes a + i (b-a)/N, cells 0 and
     N-1 Dirichlet, the cell 'centre' is its left node (line.cpp:17) and
@@ -120,6 +194,30 @@ def naca_ogrid(ni=512, nj=128, radius=20.0, first_height=2e-3):
     circle of `radius` chords centred at mid chord, geometric stretching from
     `first_height`.  The profile ring is a slip wall (the reference's 'noslip'
     ghost, expliciteuler.cpp:25-43); the outer ring is Dirichlet far field."""
+    X, Y = naca_ogrid_nodes(ni, nj, radius, first_height)
+    return _quad_mesh(X, Y, True, native.BC_NOSLIP, native.BC_DIRICHLET)
+
+
+def naca_ogrid_su2(path, ni=64, nj=16, radius=20.0, first_height=2e-3):
+    """The nodes of naca_ogrid as an SU2 triangle mesh (each quadrilateral
+    split along alternating diagonals), markers 'airfoil' (profile ring) and
+    'farfield' (outer ring).  Returns (nodes, cells)."""
+    X, Y = naca_ogrid_nodes(ni, nj, radius, first_height)
+    nodes = np.stack([X.ravel(), Y.ravel()], 1)  # node (i, j) at i*(nj+1) + j
+    nid = lambda i, j: (i % ni) * (nj + 1) + j
+    cells = []
+    for j in range(nj):
+        for i in range(ni):
+            a, b, c, d = nid(i, j), nid(i + 1, j), nid(i + 1, j + 1), nid(i, j + 1)
+            cells += [(a, b, c), (a, c, d)] if (i + j) % 2 == 0 else [(a, b, d), (b, c, d)]
+    markers = {"airfoil": [(nid(i, 0), nid(i + 1, 0)) for i in range(ni)],
+               "farfield": [(nid(i, nj), nid(i + 1, nj)) for i in range(ni)]}
+    write_su2(path, nodes, cells, markers)
+    return nodes, np.asarray(cells)
+
+
+def naca_ogrid_nodes(ni=512, nj=128, radius=20.0, first_height=2e-3):
+    """Node coordinates X, Y (ni x nj+1, periodic in i) of naca_ogrid."""
     half = ni // 2
     th = np.linspace(0.0, np.pi, half + 1)
     # half cosine, half uniform spacing: refined at both edges without
@@ -146,4 +244,4 @@ def naca_ogrid(ni=512, nj=128, radius=20.0, first_height=2e-3):
     s /= s[-1]
     X = sx[:, None] + (fx - sx)[:, None] * s[None, :]
     Y = sy[:, None] + (fy - sy)[:, None] * s[None, :]
-    return _quad_mesh(X, Y, True, native.BC_NOSLIP, native.BC_DIRICHLET)
+    return X, Y

@@ -84,6 +84,8 @@ SIGNATURES = {
     "uqk_set_refinement": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_double, C.c_double]),
     "uqk_refine": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "uqk_set_refinement_cap": (C.c_int, [C.c_void_p, C.c_int32]),
+    "uqk_su2_mesh": (C.c_int64, [C.c_char_p, C.c_char_p, C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int32]),
     "uqk_run_steps": (C.c_int64, [C.c_void_p, C.POINTER(ProblemDesc), C.POINTER(MeshDesc), C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
                                   C.c_int64, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64),
