@@ -310,6 +310,18 @@ typedef struct {
  * Returns 0, or -1 with the message in err. */
 int uqk_run_config(const char* toml_path, const char* mean_var_csv, uqk_run_report* report, char* err,
                    int32_t errlen);
+/* uqk_run_config that also writes the run's files when output_prefix is not
+ * NULL (the end of MomentSolver::Solve, momentsolver.cpp:190-228):
+ * <prefix>.vtk, the E/Var fields as cell and point data (Mesh2D::Export,
+ * mesh2d.cpp:399-615; 1D: Mesh1D::Export, mesh1d.cpp:90-220, plus
+ * <prefix>ExpectedValue and <prefix>Variance); <prefix>, a run log echoing the
+ * input (main.cpp:160-180); <prefix>_moments and <prefix>_duals, one line per
+ * cell (MomentSolver::Export, momentsolver.cpp:565-588).  An input with
+ * [general] restartFile = <prefix> (and importDualState = true for the duals)
+ * continues such a run from its previous tEnd (momentsolver.cpp:418-523),
+ * with the same moments and quadrature. */
+int uqk_run_config_output(const char* toml_path, const char* mean_var_csv, const char* output_prefix,
+                          uqk_run_report* report, char* err, int32_t errlen);
 /* SU2 mesh reader (Mesh2D::LoadSU2MeshFromFile, mesh2d.cpp:68-255, with the
  * geometry of triangle.cpp / quadrangle.cpp / cell.cpp).  bc_spec types the
  * markers: "tag=type;tag=type" with type dirichlet | noslip | swwall |

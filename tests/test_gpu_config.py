@@ -103,3 +103,124 @@ def test_euler2d_su2_triangles_input(oracle, tmp_path):
     print("report", rep, "rel L2 E/Var", err, flush=True)
     assert rep["steps"] == 10 and rep["n_cells"] == 2048
     assert err.max() < 1e-8
+
+
+def _read_vtk(path):
+    """A legacy ASCII VTK unstructured grid: points, cells, types and the
+    FIELD arrays of CELL_DATA and POINT_DATA."""
+    tok = open(path).read().split()
+    i = tok.index("POINTS")
+    P = int(tok[i + 1])
+    pts = np.array(tok[i + 3:i + 3 + 3 * P], float).reshape(P, 3)
+    i = tok.index("CELLS")
+    N = int(tok[i + 1])
+    j, cells = i + 3, []
+    for _ in range(N):
+        k = int(tok[j])
+        cells.append([int(v) for v in tok[j + 1:j + 1 + k]])
+        j += k + 1
+    i = tok.index("CELL_TYPES")
+    types = [int(v) for v in tok[i + 2:i + 2 + N]]
+    data = {}
+    for sect in ("CELL_DATA", "POINT_DATA"):
+        i = tok.index(sect)
+        nf = int(tok[i + 4])
+        j = i + 5
+        for _ in range(nf):
+            name, comp, n = tok[j], int(tok[j + 1]), int(tok[j + 2])
+            data[(sect, name)] = np.array(tok[j + 4:j + 4 + comp * n], float).reshape(n, comp)
+            j += 4 + comp * n
+    return pts, cells, types, data
+
+
+def _read_field(path, n, M, S):
+    rows = [l.rstrip("\n").split(",") for l in open(path)]
+    assert len(rows) == n and all(r[-1] == "" for r in rows)
+    x = np.array([r[:-1] for r in rows], float).reshape(n, S, M)
+    return np.ascontiguousarray(x.transpose(0, 2, 1))
+
+
+def test_output_files_and_restart(oracle, tmp_path):
+    """Euler 2D on a 32 x 8 O-grid: a run to tEnd 2e-5 writes the fields
+    (VTK cell data equal to the mean/variance, point data their averages over
+    each node's cells), the run log and the restart files; an input with
+    restartFile = that run and importDualState continues from tEnd for 5
+    steps and matches the oracle started from the same files."""
+    text = open(os.path.join(INPUT, "euler2d_naca.toml")).read()
+    for a, b in [("ni = 512", "ni = 32"), ("nj = 128", "nj = 8"), ("tEnd = 1e3", "tEnd = 2e-5")]:
+        assert a in text
+        text = text.replace(a, b)
+    p1 = tmp_path / "run1.toml"
+    p1.write_text(text)
+    pre = str(tmp_path / "out" / "run1")
+    os.makedirs(tmp_path / "out")
+    csv = str(tmp_path / "mv1.csv")
+    print("case: output files", flush=True)
+    rep1 = native.run_config(str(p1), csv, pre)
+    assert rep1["t"] >= 2e-5 and rep1["steps"] >= 4
+    n, S, M = 256, 4, 10
+    mv = np.loadtxt(csv, delimiter=",", ndmin=2)
+    pts, cells, types, data = _read_vtk(pre + ".vtk")
+    m = meshes.naca_ogrid(32, 8)
+    assert len(cells) == n and set(types) == {9} and pts.shape == (32 * 9, 3)
+    np.testing.assert_allclose(pts[np.array(cells)].mean(1)[:, :2], m.center, atol=1e-12)
+    rho, mom = data[("CELL_DATA", "E(%CF%81)")], data[("CELL_DATA", "Var(%CF%81U)")]
+    np.testing.assert_array_equal(rho[:, 0], mv[:, 0])
+    np.testing.assert_array_equal(mom[:, :2], mv[:, 5:7])
+    assert not mom[:, 2].any()
+    users = np.zeros(len(pts))
+    acc = np.zeros(len(pts))
+    for c, nodes in enumerate(cells):
+        users[nodes] += 1
+        acc[nodes] += mv[c, 3]
+    np.testing.assert_allclose(data[("POINT_DATA", "E(%CF%81E)")][:, 0], acc / users, rtol=1e-14)
+    log = open(pre).read()
+    assert "Config file:\t" + str(p1) in log and "ni = 32" in log
+    u1 = _read_field(pre + "_moments", n, M, S)
+    l1 = _read_field(pre + "_duals", n, M, S)
+
+    # continue: restartFile + importDualState, tEnd 1e3, 5 more steps
+    text2 = text.replace("tEnd = 2e-5", "tEnd = 1e3").replace("maxSteps = 2000", "maxSteps = 5").replace(
+        '[general]', '[general]\nrestartFile = "out/run1"\nimportDualState = true')
+    p2 = tmp_path / "run2.toml"
+    p2.write_text(text2)
+    inf, u0, lam0, _ = native.config_setup(str(p2))
+    np.testing.assert_array_equal(u0, u1)
+    np.testing.assert_array_equal(lam0, l1)
+    csv2 = str(tmp_path / "mv2.csv")
+    rep2 = native.run_config(str(p2), csv2)
+    assert rep2["steps"] == 5 and rep2["t"] > 2e-5
+    b = native.Basis(2, [3], 5)
+    c = _oracle_closure(oracle, native.EULER2D, eps=1e-6)
+    oprob = oracle.problem(oracle.P_EULER2D, flux=oracle.FLUX_LF, cfl=0.4)
+    uo, lo = u1.copy(), l1.copy()
+    res = oracle.run(c, oprob, _oracle_mesh(oracle, m), b.phi, b.phiw, b.xi, uo, lo, 1e3, 5)
+    assert res["rc"] == 0 and res["steps"] == 5
+    assert rep2["t"] == pytest.approx(2e-5 + res["t"], rel=1e-12)
+    err = _rel_l2(np.loadtxt(csv2, delimiter=",", ndmin=2), oracle.mean_var(c, b.phi, b.phiw, lo), m.area)
+    print("restart rel L2", err, flush=True)
+    assert err.max() < 1e-8
+
+
+def test_output_files_1d(tmp_path):
+    """Burgers: the strip of quadrilaterals (two rows of N+1 points, a tenth of
+    the domain high) and the ExpectedValue / Variance text files"""
+    text = open(os.path.join(INPUT, "burgers1d_step.toml")).read()
+    text = text.replace("numberOfCells = 2000", "numberOfCells = 50").replace("tEnd = 0.11", "tEnd = 0.01")
+    p = tmp_path / "b.toml"
+    p.write_text(text)
+    pre = str(tmp_path / "burgers")
+    csv = str(tmp_path / "mv.csv")
+    print("case: 1D output files", flush=True)
+    native.run_config(str(p), csv, pre)
+    mv = np.loadtxt(csv, delimiter=",", ndmin=2)
+    pts, cells, types, data = _read_vtk(pre + ".vtk")
+    assert pts.shape == (102, 3) and len(cells) == 50 and set(types) == {9}
+    assert pts[:51, 1].max() == 0.0 and np.allclose(pts[51:, 1], 0.3)
+    np.testing.assert_array_equal(data[("CELL_DATA", "E(u)")][:, 0], mv[:, 0])
+    ev = np.loadtxt(pre + "ExpectedValue")
+    var = np.loadtxt(pre + "Variance")
+    assert ev.shape == var.shape == (50, 2)
+    np.testing.assert_allclose(ev[:, 0], np.arange(50) * 3.0 / 50, atol=1e-12)
+    np.testing.assert_allclose(ev[:, 1], mv[:, 0], rtol=1e-5)
+    np.testing.assert_allclose(var[:, 1], mv[:, 1], rtol=1e-5, atol=1e-12)

@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <sstream>
+#include <fstream>
 #include <stdexcept>
 
 namespace uqk_host {
@@ -32,9 +34,15 @@ int closure_kind(const std::string& s) {
 }  // namespace
 
 Settings Settings::from_file(const std::string& path) {
-    Settings s = from_toml(Toml::parse_file(path));
+    std::ifstream f(path);
+    if (!f) throw std::runtime_error("cannot open " + path);
+    std::stringstream text;
+    text << f.rdbuf();
+    Settings s = from_toml(Toml::parse(text.str()));
     const size_t slash = path.find_last_of('/');
     s.input_dir = slash == std::string::npos ? "." : path.substr(0, slash);
+    s.input_path = path;
+    s.input_text = text.str();
     return s;
 }
 
@@ -44,6 +52,8 @@ Settings Settings::from_toml(const Toml& t) {
     if (s.problem != "Burgers1D" && s.problem != "Euler2D" && s.problem != "ShallowWater2D")
         throw std::runtime_error("[general] 'problem' must be Burgers1D, Euler2D or ShallowWater2D");
     s.output_dir = t.get_string("general", "outputDir", ".");
+    s.restart_file = t.get_string("general", "restartFile", "");  // settings.cpp:74-79
+    s.import_dual_state = t.get_bool("general", "importDualState", false);
     s.dimension = (int)t.get_number("mesh", "dimension", s.problem == "Burgers1D" ? 1 : 2);
     s.mesh_type = t.get_string("mesh", "type", s.dimension == 1 ? "line" : "nacaOGrid");
     if (t.get_string("mesh", "format", "") == "SU2") {  // mesh2d.cpp:3-61
@@ -163,6 +173,14 @@ Mesh Mesh::line(long n, double a, double b) {
         m.unit_normal[2 * j + 1] = m.normal[2 * j + 1] = 1.0;
     }
     m.bc[0] = m.bc[n - 1] = UQK_BC_DIRICHLET;
+    m.cell_node_count = 2;
+    m.node_xy.assign(2 * (size_t)(n + 1), 0.0);
+    m.cell_nodes.resize(2 * (size_t)n);
+    for (long i = 0; i <= n; ++i) m.node_xy[2 * i] = a + i * (b - a) / n;
+    for (long j = 0; j < n; ++j) {
+        m.cell_nodes[2 * j] = j;
+        m.cell_nodes[2 * j + 1] = j + 1;
+    }
     return m;
 }
 
@@ -189,6 +207,13 @@ Mesh Mesh::quad(const std::vector<double>& X, const std::vector<double>& Y, int 
     m.dx_param.resize(N);
     m.bnormal.resize(2 * N);
     m.center.resize(2 * N);
+    m.cell_node_count = 4;
+    m.node_xy.resize(2 * (size_t)ni_nodes * nj_nodes);
+    for (size_t k = 0; k < (size_t)ni_nodes * nj_nodes; ++k) {
+        m.node_xy[2 * k] = X[k];
+        m.node_xy[2 * k + 1] = Y[k];
+    }
+    m.cell_nodes.resize(4 * (size_t)N);
     auto node = [&](int i, int j, double& x, double& y) {
         x = X[(size_t)i * nj_nodes + j];
         y = Y[(size_t)i * nj_nodes + j];
@@ -202,6 +227,10 @@ Mesh Mesh::quad(const std::vector<double>& X, const std::vector<double>& Y, int 
             node(ip, j, px[1], py[1]);
             node(ip, j + 1, px[2], py[2]);
             node(i, j + 1, px[3], py[3]);
+            m.cell_nodes[4 * c + 0] = (int64_t)i * nj_nodes + j;
+            m.cell_nodes[4 * c + 1] = (int64_t)ip * nj_nodes + j;
+            m.cell_nodes[4 * c + 2] = (int64_t)ip * nj_nodes + j + 1;
+            m.cell_nodes[4 * c + 3] = (int64_t)i * nj_nodes + j + 1;
             const double cx = 0.25 * (px[0] + px[1] + px[2] + px[3]), cy = 0.25 * (py[0] + py[1] + py[2] + py[3]);
             double ar = 0.0, lmax = 0.0;
             for (int e = 0; e < 4; ++e) {

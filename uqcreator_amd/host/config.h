@@ -16,12 +16,15 @@ struct Settings {
     // [general]
     std::string problem;          // Burgers1D | Euler2D | ShallowWater2D
     std::string output_dir = ".";
+    std::string restart_file;        // restartFile: a run log with _moments / _duals beside it
+    bool import_dual_state = false;  // importDualState
     // [mesh]
     int dimension = 1;
     std::string mesh_type;        // line | nacaOGrid | quadGrid | SU2 (format = "SU2")
     std::string su2_file;         // SU2MeshFile, relative to the input file's directory
     std::vector<std::pair<std::string, std::string>> su2_bc;  // SU2BC: marker tag -> type
     std::string input_dir = ".";
+    std::string input_path, input_text;  // the file itself, for the run log
     long n_cells_1d = 0;
     double a = 0.0, b = 0.0;
     int ni = 512, nj = 128;
@@ -66,6 +69,11 @@ struct Mesh {
     int n_edges = 0, dim = 0;
     std::vector<int32_t> nbr, bc;
     std::vector<double> unit_normal, normal, area, dx_param, bnormal, center;
+    // geometry for output files: node coordinates (x, y) and the nodes of each
+    // cell (cell_node_count per cell, counter-clockwise or as read)
+    std::vector<double> node_xy;
+    std::vector<int64_t> cell_nodes;
+    int cell_node_count = 0;
 
     static Mesh line(long n, double a, double b);
     static Mesh quad(const std::vector<double>& X, const std::vector<double>& Y, int ni_nodes, int nj_nodes,

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "config.h"
+#include "io.h"
 #include "uqk.h"
 #include "uqk_host.h"
 
@@ -173,14 +174,51 @@ struct RunOutput {
     int S = 0, M = 0, Q = 0;
 };
 
-static RunOutput run_config(const std::string& path, const char* mean_var_csv) {
+// The start of a run: moments and duals, the start time, and whether the
+// duals must be converged first.  Without a restart file: SetupIC and
+// SetDuals' (DS(mean), 0, ...).  With one (DetermineMoments,
+// DeterminePreviousSettings, SetDuals; momentsolver.cpp:418-523): the
+// moments of <restartFile>_moments, the duals of <restartFile>_duals when
+// importDualState is set, and the previous run's tEnd as the start time.
+// A restart keeps the previous basis and quadrature (the reference can also
+// raise the order on restart; that path is not provided here).
+struct Start {
+    std::vector<double> u, lam;
+    double t0 = 0.0;
+    bool presolve = false;
+};
+
+static Start start_of(const Settings& s, const Mesh& m, const Basis& b) {
+    Start st;
+    const long N = m.n_cells;
+    const int S = s.n_states, M = b.M;
+    if (s.restart_file.empty()) {
+        st.u = setup_ic(s, m, b);
+        st.lam = set_duals(s, st.u, N, M);
+        st.presolve = s.max_iterations == 1;
+        return st;
+    }
+    const std::string rf = s.restart_file[0] == '/' ? s.restart_file : s.input_dir + "/" + s.restart_file;
+    const Settings prev = Settings::from_toml(Toml::parse(read_log_config(rf)));
+    if (prev.degrees != s.degrees || prev.max_degree != s.max_degree || prev.quad_points != s.quad_points ||
+        prev.dim_xi != s.dim_xi)
+        throw std::runtime_error("restart: the previous run used other moments or quadrature (not supported)");
+    st.u = read_restart_field(rf + "_moments", N, S, M);
+    st.lam = s.import_dual_state ? read_restart_field(rf + "_duals", N, S, M) : set_duals(s, st.u, N, M);
+    st.t0 = prev.t_end;
+    st.presolve = s.max_iterations == 1 && !s.import_dual_state;
+    return st;
+}
+
+static RunOutput run_config(const std::string& path, const char* mean_var_csv, const char* output_prefix) {
     const Settings s = Settings::from_file(path);
     const Basis b = make_basis(s);
     const Mesh m = Mesh::from_settings(s);
     const int S = s.n_states, M = b.M, Q = b.Q;
     const long N = m.n_cells;
-    std::vector<double> u = setup_ic(s, m, b);
-    std::vector<double> lam = set_duals(s, u, N, M);
+    Start start = start_of(s, m, b);
+    std::vector<double>& u = start.u;
+    std::vector<double>& lam = start.lam;
 
     RunOutput out;
     out.n_cells = N;
@@ -206,7 +244,7 @@ static RunOutput run_config(const std::string& path, const char* mean_var_csv) {
     prob.friction = s.friction;
     prob.friction_dim = s.friction_dim;
 
-    if (s.max_iterations == 1) {
+    if (start.presolve) {
         // one-shot IPM starts from converged duals (momentsolver.cpp:487-493)
         uqk_ctx* c0 = make_ctx(s, b, 10000, true);
         int64_t f = uqk_closure_solve(c0, N, nullptr, du.as<double>(), dl.as<double>(), nullptr, nullptr);
@@ -246,7 +284,7 @@ static RunOutput run_config(const std::string& path, const char* mean_var_csv) {
             if (uqk_memcpy(nt, nt0.data(), (int64_t)(sizeof(int32_t) * nt0.size()), 0) != 0)
                 throw std::runtime_error("host to device copy failed");
         }
-        double t = 0.0, res = s.min_residual + 1.0;
+        double t = start.t0, res = s.min_residual + 1.0;
         while (t < s.t_end && (res > s.min_residual || (retard && rl < s.retardation.size())) &&
                (s.max_steps < 0 || out.steps < s.max_steps)) {
             double dt = 0.0;
@@ -268,24 +306,44 @@ static RunOutput run_config(const std::string& path, const char* mean_var_csv) {
         out.residual = res;
     } else {
         nfail = uqk_run_steps(ctx, &prob, &mesh, nt, du.as<double>(), dl.as<double>(), duq.as<double>(),
-                              dus.as<double>(), dst.as<int32_t>(), dit.as<int32_t>(), 0.0, s.t_end, s.max_steps, 1,
+                              dus.as<double>(), dst.as<int32_t>(), dit.as<int32_t>(), start.t0, s.t_end, s.max_steps, 1,
                               &out.t, &out.steps, &out.residual, &out.newton);
     }
     auto toc = std::chrono::steady_clock::now();
     out.runtime_s = std::chrono::duration<double>(toc - tic).count();
     if (nfail < 0) throw std::runtime_error(uqk_last_error(ctx));
     if (nfail > 0) throw std::runtime_error("[closure] Newton failed in " + std::to_string(nfail) + " cells");
+    if (!mean_var_csv && !output_prefix) return out;
+    std::vector<double> h((size_t)N * 2 * S);
+    {
+        DeviceBuf mv(sizeof(double) * h.size());
+        if (uqk_mean_var(ctx, N, nt, dl.as<double>(), mv.as<double>()) != 0) throw std::runtime_error(uqk_last_error(ctx));
+        if (uqk_memcpy(h.data(), mv.p, (int64_t)(sizeof(double) * h.size()), 1) != 0)
+            throw std::runtime_error("device to host copy failed");
+    }
     if (mean_var_csv) {
-        DeviceBuf mv(sizeof(double) * (size_t)N * 2 * S);
-        uqk_mean_var(ctx, N, nt, dl.as<double>(), mv.as<double>());
-        std::vector<double> h((size_t)N * 2 * S);
-        uqk_memcpy(h.data(), mv.p, (int64_t)(sizeof(double) * h.size()), 1);
         std::ofstream f(mean_var_csv);
         f.precision(17);
         for (long j = 0; j < N; ++j) {
             for (int q = 0; q < 2 * S; ++q) f << (q ? "," : "") << h[(size_t)j * 2 * S + q];
             f << "\n";
         }
+        if (!f) throw std::runtime_error(std::string("write failed: ") + mean_var_csv);
+    }
+    if (output_prefix) {
+        // the end of MomentSolver::Solve (momentsolver.cpp:221-228): fields,
+        // then the moments and duals a restart reads beside the log
+        const std::string pre = output_prefix;
+        write_fields(pre, s, m, h);
+        if (uqk_memcpy(u.data(), du.p, (int64_t)(sizeof(double) * u.size()), 1) != 0 ||
+            uqk_memcpy(lam.data(), dl.p, (int64_t)(sizeof(double) * lam.size()), 1) != 0)
+            throw std::runtime_error("device to host copy failed");
+        char line[160];
+        std::snprintf(line, sizeof line, "steps %lld  t %.17g  residual %.6e  runtime %.3f s", (long long)out.steps,
+                      out.t, out.residual, out.runtime_s);
+        write_log(pre, s.input_path, s.input_text, {line});
+        write_restart_field(pre + "_moments", u, N, S, M);
+        write_restart_field(pre + "_duals", lam, N, S, M);
     }
     return out;
 }
@@ -296,8 +354,13 @@ extern "C" {
 
 int uqk_run_config(const char* toml_path, const char* mean_var_csv, uqk_run_report* report, char* err,
                    int32_t errlen) {
+    return uqk_run_config_output(toml_path, mean_var_csv, nullptr, report, err, errlen);
+}
+
+int uqk_run_config_output(const char* toml_path, const char* mean_var_csv, const char* output_prefix,
+                          uqk_run_report* report, char* err, int32_t errlen) {
     try {
-        uqk_host::RunOutput o = uqk_host::run_config(toml_path, mean_var_csv);
+        uqk_host::RunOutput o = uqk_host::run_config(toml_path, mean_var_csv, output_prefix);
         if (report) {
             report->n_cells = o.n_cells;
             report->steps = o.steps;
@@ -337,12 +400,9 @@ int64_t uqk_config_setup(const char* toml_path, int32_t* info, double* u, double
             info[7] = s.precision;
         }
         if (u || lambda) {
-            std::vector<double> uu = uqk_host::setup_ic(s, m, b);
-            if (u) std::copy(uu.begin(), uu.end(), u);
-            if (lambda) {
-                std::vector<double> ll = uqk_host::set_duals(s, uu, m.n_cells, b.M);
-                std::copy(ll.begin(), ll.end(), lambda);
-            }
+            const uqk_host::Start st = uqk_host::start_of(s, m, b);
+            if (u) std::copy(st.u.begin(), st.u.end(), u);
+            if (lambda) std::copy(st.lam.begin(), st.lam.end(), lambda);
         }
         if (center) std::copy(m.center.begin(), m.center.end(), center);
         return m.n_cells;

@@ -170,6 +170,9 @@ Mesh Mesh::su2(const std::string& path, const std::vector<std::pair<std::string,
         for (int i = 0; i < k; ++i) cn.push_back(node(ids[i]));
     }
     m.n_edges = E;
+    m.cell_node_count = E;
+    m.cell_nodes.assign(cn.begin(), cn.end());
+    m.node_xy = xy;
     const size_t N = (size_t)nc;
     m.nbr.assign(N * E, (int32_t)nc);
     m.unit_normal.assign(N * E * 2, 0.0);

@@ -84,6 +84,8 @@ SIGNATURES = {
     "uqk_set_refinement": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_double, C.c_double]),
     "uqk_refine": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "uqk_set_refinement_cap": (C.c_int, [C.c_void_p, C.c_int32]),
+    "uqk_run_config_output": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(RunReport), C.c_char_p,
+                                        C.c_int32]),
     "uqk_su2_mesh": (C.c_int64, [C.c_char_p, C.c_char_p, C.POINTER(C.c_int32), C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int32]),
     "uqk_run_steps": (C.c_int64, [C.c_void_p, C.POINTER(ProblemDesc), C.POINTER(MeshDesc), C.c_void_p, C.c_void_p,
@@ -469,13 +471,14 @@ def config_setup(path):
     return inf, u, lam, center
 
 
-def run_config(path, mean_var_csv=None):
-    """Runs an input file end to end on the GPU (uqk_run_config); returns the
-    run report as a dict."""
+def run_config(path, mean_var_csv=None, output_prefix=None):
+    """Runs an input file end to end on the GPU (uqk_run_config_output);
+    returns the run report as a dict.  output_prefix: write <prefix>.vtk, the
+    run log <prefix> and the restart files <prefix>_moments / _duals."""
     err = C.create_string_buffer(1024)
     rep = RunReport()
-    rc = lib().uqk_run_config(path.encode(), mean_var_csv.encode() if mean_var_csv else None, C.byref(rep), err,
-                              1024)
+    rc = lib().uqk_run_config_output(path.encode(), mean_var_csv.encode() if mean_var_csv else None,
+                                     output_prefix.encode() if output_prefix else None, C.byref(rep), err, 1024)
     if rc != 0:
         raise RuntimeError(err.value.decode())
     return {k: getattr(rep, k) for k, _ in RunReport._fields_}

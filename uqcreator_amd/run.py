@@ -2,10 +2,13 @@
 report as one JSON line.
 
     python -m uqcreator_amd.run INPUT.toml [--max-steps N] [--precision fp64|mixed]
-                                            [--mean-var OUT.csv]
+                                            [--mean-var OUT.csv] [--output PREFIX]
 
 --max-steps and --precision override the file's [problem] maxSteps and
 [moment_system] precision (a copy of the file with those keys replaced is run).
+--output writes PREFIX.vtk (mean and variance fields), the run log PREFIX and
+the restart files PREFIX_moments / PREFIX_duals; an input with [general]
+restartFile = "PREFIX" continues from them.
 """
 import argparse
 import json
@@ -41,6 +44,7 @@ def main(argv=None):
     ap.add_argument("--max-steps", type=int)
     ap.add_argument("--precision", choices=["fp64", "mixed"])
     ap.add_argument("--mean-var")
+    ap.add_argument("--output")
     args = ap.parse_args(argv)
     path = args.input
     tmp = None
@@ -50,12 +54,13 @@ def main(argv=None):
             text = _override(text, "problem", "maxSteps", str(args.max_steps))
         if args.precision:
             text = _override(text, "moment_system", "precision", '"%s"' % args.precision)
-        fd, tmp = tempfile.mkstemp(suffix=".toml")
+        # beside the input, so that paths in it (SU2MeshFile, restartFile) resolve alike
+        fd, tmp = tempfile.mkstemp(suffix=".toml", dir=os.path.dirname(os.path.abspath(path)))
         with os.fdopen(fd, "w") as f:
             f.write(text)
         path = tmp
     try:
-        rep = native.run_config(path, args.mean_var)
+        rep = native.run_config(path, args.mean_var, args.output)
     finally:
         if tmp:
             os.unlink(tmp)
