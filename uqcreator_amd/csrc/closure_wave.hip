@@ -440,12 +440,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                         else if (cc == N && r < N) yb[r] = T[ra][i][tb];
                     }
             wave_sync();
+            // U x = y with lane r owning x_r: row r of U and 1/U_rr are read
+            // into registers first, so the chain of a step is one multiply,
+            // one SGPR broadcast and one FMA.
             double yr = lane < N ? yb[lane] : 0.0;
+            const double rd = lane < N ? rdiag[lane] : 0.0;
+            double ucol[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) ucol[i] = lane < i ? Hs[i * (i + 1) / 2 + lane] : 0.0;
 #pragma unroll
             for (int i = N - 1; i >= 0; --i) {
-                const double xi = readlane_f64(yr, i) * rdiag[i];
-                if (lane < i) yr -= Hs[i * (i + 1) / 2 + lane] * xi;
-                if (lane == i) yr = xi;
+                const double xi = readlane_f64(yr * rd, i);
+                yr = lane == i ? xi : fma(-ucol[i], xi, yr);
             }
             d = yr;
             wave_sync();
