@@ -410,20 +410,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                     uc0[tb] = urow[8 * tb + q8];
                     if (two) uc1[tb] = urow[64 + 8 * tb + q8];
                 }
+                // Only rows past the pivots must keep their values; entries
+                // below the diagonal or past row N-1 / column N are never read
+                // (pivots, published rows and the solve take r <= c), so they
+                // are updated along without a per-entry test.
                 const int jl = two ? j + 1 : j;  // last eliminated pivot
 #pragma unroll
                 for (int ra = aj; ra < RA; ++ra)
 #pragma unroll
-                    for (int i = 0; i < 2; ++i)
+                    for (int i = 0; i < 2; ++i) {
+                        const int r = 16 * ra + 2 * p8 + i;
+                        if (r > jl) {
 #pragma unroll
-                        for (int tb = j >> 3; tb < NS; ++tb) {
-                            const int r = 16 * ra + 2 * p8 + i, cc = 8 * tb + q8;
-                            if (r > jl && cc > jl && r <= cc && r < N && cc <= N) {
+                            for (int tb = j >> 3; tb < NS; ++tb) {
                                 double t = T[ra][i][tb] - ur0[ra][i] * uc0[tb];
                                 if (two) t -= ur1[ra][i] * uc1[tb];
                                 T[ra][i][tb] = t;
                             }
                         }
+                    }
             }
             if (!ok) return false;
             // phase end: factor
