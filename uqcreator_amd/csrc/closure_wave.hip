@@ -363,8 +363,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                     break;
                 }
                 const double r0 = rsqrt_f64(h00);
-                const double u00 = h00 * r0;
-                double r1 = 0.0, u01 = 0.0, u11 = 0.0;
+                double r1 = 0.0, u01 = 0.0;
                 if (two) {
                     u01 = readlane_f64(T[aj][0][(j + 1) >> 3], l1) * r0;
                     const double h11 = readlane_f64(T[aj][1][(j + 1) >> 3], l1) - u01 * u01;
@@ -373,7 +372,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                         break;
                     }
                     r1 = rsqrt_f64(h11);
-                    u11 = h11 * r1;
                 }
                 if (lane == 0) {
                     rdiag[j] = r0;
@@ -381,18 +379,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                 }
                 double* urow = rowb + 128 * ((j >> 1) & 1);  // rows j, j+1 at [c], [64 + c]
                 if (p8 == pj) {
+                    // the same formulas give the diagonal (U_jj = h00 r0,
+                    // U_j+1,j+1 = (h11 - u01^2) r1); columns left of the pivots
+                    // get values below the diagonal, which nothing reads
 #pragma unroll
                     for (int tb = j >> 3; tb < NS; ++tb) {
                         const int cc = 8 * tb + q8;
-                        if (cc >= j) {
-                            const double v0 = (cc == j) ? u00 : T[aj][0][tb] * r0;
-                            T[aj][0][tb] = v0;
-                            urow[cc] = v0;
-                            if (two && cc > j) {
-                                const double v1 = (cc == j + 1) ? u11 : (T[aj][1][tb] - u01 * v0) * r1;
-                                T[aj][1][tb] = v1;
-                                urow[64 + cc] = v1;
-                            }
+                        const double v0 = T[aj][0][tb] * r0;
+                        T[aj][0][tb] = v0;
+                        urow[cc] = v0;
+                        if (two) {
+                            const double v1 = (T[aj][1][tb] - u01 * v0) * r1;
+                            T[aj][1][tb] = v1;
+                            urow[64 + cc] = v1;
                         }
                     }
                 }
