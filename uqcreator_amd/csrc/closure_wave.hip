@@ -357,21 +357,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                 const bool two = j + 1 < N;          // the last step of an odd n has one pivot
                 const int pj = (j & 15) >> 1, aj = j >> 4;
                 const int l0 = 8 * pj + (j & 7), l1 = 8 * pj + ((j + 1) & 7);
+                // 2x2 pivot block [h00 h01; h01 h11]: U_j+1,j+1^2 = det / h00,
+                // so 1/U_j+1,j+1 = sqrt(h00) / sqrt(det) and both square roots
+                // start at once instead of one after the other
                 const double h00 = readlane_f64(T[aj][0][j >> 3], l0);
                 if (!(h00 > 0.0)) {
                     ok = false;
                     break;
                 }
-                const double r0 = rsqrt_f64(h00);
-                double r1 = 0.0, u01 = 0.0;
+                double r1 = 0.0, u01 = 0.0, rdet = 0.0;
+                double h01 = 0.0, det = 0.0;
                 if (two) {
-                    u01 = readlane_f64(T[aj][0][(j + 1) >> 3], l1) * r0;
-                    const double h11 = readlane_f64(T[aj][1][(j + 1) >> 3], l1) - u01 * u01;
-                    if (!(h11 > 0.0)) {
+                    h01 = readlane_f64(T[aj][0][(j + 1) >> 3], l1);
+                    det = fma(h00, readlane_f64(T[aj][1][(j + 1) >> 3], l1), -h01 * h01);
+                    if (!(det > 0.0)) {
                         ok = false;
                         break;
                     }
-                    r1 = rsqrt_f64(h11);
+                    rdet = rsqrt_f64(det);
+                }
+                const double r0 = rsqrt_f64(h00);
+                if (two) {
+                    u01 = h01 * r0;
+                    r1 = rdet * (h00 * r0);
                 }
                 if (lane == 0) {
                     rdiag[j] = r0;
@@ -422,6 +430,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                         if (r > jl) {
 #pragma unroll
                             for (int tb = j >> 3; tb < NS; ++tb) {
+                                if (8 * tb + 7 < 16 * ra) continue;  // slot wholly below the diagonal
                                 double t = T[ra][i][tb] - ur0[ra][i] * uc0[tb];
                                 if (two) t -= ur1[ra][i] * uc1[tb];
                                 T[ra][i][tb] = t;
