@@ -58,12 +58,13 @@ struct WaveShape {
     static constexpr int HS = UB + S * QP;       // packed H / U factor
     static constexpr int YB = HS + HP;           // 64
     static constexpr int DIAG = YB + 64;         // 64 reciprocal pivots
-    static constexpr int ROW = DIAG + 64;        // 2 x 128: rows j, j+1 of U, double buffered
+    static constexpr int ROW = (DIAG + 64 + 1) & ~1;  // 2 x 128: rows j, j+1 of U, double buffered (16-byte aligned)
     static constexpr int WAVE_DOUBLES = ROW + 256;
     static constexpr int TABLE_DOUBLES = 2 * M * QP;  // phiT, phiwT (i-major)
 };
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
 
 // PREC 0: FP64 Hessian on the vector units.  PREC 1: Hessian on MFMA
 // (v_mfma_f32_16x16x4f32, FP32 operands and accumulation); gradient, residual,
@@ -385,7 +386,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                     rdiag[j] = r0;
                     if (two) rdiag[j + 1] = r1;
                 }
-                double* urow = rowb + 128 * ((j >> 1) & 1);  // rows j, j+1 at [c], [64 + c]
+                // rows j, j+1 interleaved: (U_j,c, U_j+1,c) at urow[2c], 16-byte reads
+                d2* urow = reinterpret_cast<d2*>(rowb + 128 * ((j >> 1) & 1));
                 if (p8 == pj) {
                     // the same formulas give the diagonal (U_jj = h00 r0,
                     // U_j+1,j+1 = (h11 - u01^2) r1); columns left of the pivots
@@ -395,12 +397,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                         const int cc = 8 * tb + q8;
                         const double v0 = T[aj][0][tb] * r0;
                         T[aj][0][tb] = v0;
-                        urow[cc] = v0;
+                        double v1 = 0.0;
                         if (two) {
-                            const double v1 = (T[aj][1][tb] - u01 * v0) * r1;
+                            v1 = (T[aj][1][tb] - u01 * v0) * r1;
                             T[aj][1][tb] = v1;
-                            urow[64 + cc] = v1;
                         }
+                        urow[cc] = d2{v0, v1};
                     }
                 }
                 wave_sync();
@@ -409,13 +411,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                 for (int ra = aj; ra < RA; ++ra)
 #pragma unroll
                     for (int i = 0; i < 2; ++i) {
-                        ur0[ra][i] = urow[16 * ra + 2 * p8 + i];
-                        if (two) ur1[ra][i] = urow[64 + 16 * ra + 2 * p8 + i];
+                        const d2 v = urow[16 * ra + 2 * p8 + i];
+                        ur0[ra][i] = v.x;
+                        ur1[ra][i] = v.y;
                     }
 #pragma unroll
                 for (int tb = j >> 3; tb < NS; ++tb) {
-                    uc0[tb] = urow[8 * tb + q8];
-                    if (two) uc1[tb] = urow[64 + 8 * tb + q8];
+                    const d2 v = urow[8 * tb + q8];
+                    uc0[tb] = v.x;
+                    uc1[tb] = v.y;
                 }
                 // Only rows past the pivots must keep their values; entries
                 // below the diagonal or past row N-1 / column N are never read
