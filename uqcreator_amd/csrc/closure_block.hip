@@ -57,6 +57,13 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
 }  // namespace
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+// 1/sqrt(h) from v_rsq_f64 (about 24 bits) refined by two Newton steps
+__device__ __forceinline__ double block_rsqrt(double h) {
+    double r = __builtin_amdgcn_rsq(h);
+    r = r * (1.5 - 0.5 * h * r * r);
+    r = r * (1.5 - 0.5 * h * r * r);
+    return r;
+}
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 template <int KIND, int PREC>
@@ -294,13 +301,13 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                             }
                         }
                     }
-                    bool ok = h00 > 0.0;
-                    const double r0 = ok ? 1.0 / sqrt(h00) : 0.0;
-                    const double u00 = h00 * r0, u01 = h01 * r0;
-                    const double h11s = h11 - u01 * u01;
-                    if (two) ok = ok && h11s > 0.0;
-                    const double r1 = (two && ok) ? 1.0 / sqrt(h11s) : 0.0;
-                    const double u11 = h11s * r1;
+                    // U_j+1,j+1^2 = det / h00, so 1/U_j+1,j+1 = sqrt(h00) / sqrt(det)
+                    // and both square roots start at once
+                    const double det = fma(h00, h11, -h01 * h01);
+                    bool ok = h00 > 0.0 && (!two || det > 0.0);
+                    const double r0 = ok ? block_rsqrt(h00) : 0.0;
+                    const double u01 = h01 * r0;
+                    const double r1 = (two && ok) ? block_rsqrt(det) * (h00 * r0) : 0.0;
                     if (q16 == 0) {
                         rdiag[j] = r0;
                         if (two) rdiag[j + 1] = r1;
@@ -312,15 +319,15 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
 #pragma unroll
                         for (int tb = 0; tb < BNB; ++tb) {
                             const int cc = 16 * tb + q16;
-                            if (cc >= j && cc < n) {
-                                const double v0 = (cc == j) ? u00 : T[ta][0][tb] * r0;
-                                T[ta][0][tb] = v0;
-                                urow[cc] = v0;
-                                if (two && cc > j) {
-                                    const double v1 = (cc == j + 1) ? u11 : (T[ta][1][tb] - u01 * v0) * r1;
-                                    T[ta][1][tb] = v1;
-                                    urow[128 + cc] = v1;
-                                }
+                            // the same formulas give the diagonal; columns left of
+                            // the pivots or past n get values nothing reads
+                            const double v0 = T[ta][0][tb] * r0;
+                            T[ta][0][tb] = v0;
+                            urow[cc] = v0;
+                            if (two) {
+                                const double v1 = (T[ta][1][tb] - u01 * v0) * r1;
+                                T[ta][1][tb] = v1;
+                                urow[128 + cc] = v1;
                             }
                         }
                     }
@@ -342,12 +349,14 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
 #pragma unroll
                     for (int i = 0; i < 2; ++i) {
                         const int r = 32 * ta + 2 * p16 + i;
-                        if (r <= jl || r >= n) continue;
+                        // only rows past the pivots must keep their values;
+                        // entries below the diagonal or past n are never read
+                        if (r <= jl) continue;
                         const double ur0 = urow[r], ur1 = two ? urow[128 + r] : 0.0;
 #pragma unroll
                         for (int tb = 0; tb < BNB; ++tb) {
-                            const int cc = 16 * tb + q16;
-                            if (r <= cc) T[ta][i][tb] -= ur0 * uc0[tb] + ur1 * uc1[tb];
+                            if (16 * tb + 15 < 32 * ta) continue;  // slot wholly below the diagonal
+                            T[ta][i][tb] -= ur0 * uc0[tb] + ur1 * uc1[tb];
                         }
                     }
                 }
