@@ -271,6 +271,9 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
             // publish them in LDS; after one barrier every thread applies the
             // rank-2 update to its tiles.
             const int p16 = tid >> 4, q16 = tid & 15;
+            // with a free column the right-hand side rides along as column n:
+            // its elimination is the forward substitution U^T y = g
+            const bool aug = n < 16 * BNB;
             double T[BRA][2][BNB];
 #pragma unroll
             for (int ta = 0; ta < BRA; ++ta)
@@ -279,7 +282,7 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
 #pragma unroll
                     for (int tb = 0; tb < BNB; ++tb) {
                         const int r = 32 * ta + 2 * p16 + i, cc = 16 * tb + q16;
-                        T[ta][i][tb] = (r <= cc && cc < n) ? H[bpidx(r, cc)] : 0.0;
+                        T[ta][i][tb] = (r <= cc && cc < n) ? H[bpidx(r, cc)] : (aug && cc == n && r < n) ? g[r] : 0.0;
                     }
             for (int j = 0; j < n; j += 2) {
                 const bool two = j + 1 < n;
@@ -339,7 +342,7 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
 #pragma unroll
                 for (int tb = 0; tb < BNB; ++tb) {
                     const int cc = 16 * tb + q16;
-                    const bool live = cc > jl && cc < n;
+                    const bool live = cc > jl && cc <= n;  // column n: the right-hand side
                     uc0[tb] = live ? urow[cc] : 0.0;
                     uc1[tb] = (live && two) ? urow[128 + cc] : 0.0;
                 }
@@ -369,17 +372,22 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
 #pragma unroll
                     for (int tb = 0; tb < BNB; ++tb) {
                         const int r = 32 * ta + 2 * p16 + i, cc = 16 * tb + q16;
-                        if (r <= cc && cc < n) H[bpidx(r, cc)] = T[ta][i][tb];
+                        if (r <= cc && cc < n)
+                            H[bpidx(r, cc)] = T[ta][i][tb];
+                        else if (aug && cc == n && r < n)
+                            dv[r] = T[ta][i][tb];
                     }
-            for (int e = tid; e < n; e += kBlockThreads) dv[e] = g[e];
+            if (!aug)
+                for (int e = tid; e < n; e += kBlockThreads) dv[e] = g[e];
             __syncthreads();
             // phase end: factor
             // phase: backsub
-            // U^T y = g then U x = y, in blocks of 16 unknowns: the 16 x 16
-            // diagonal block is solved by lanes 0..15 of wavefront 0 (SGPR
-            // broadcasts), the rest of the vector is updated by all threads.
+            // U^T y = g (unless eliminated with the factor) then U x = y, in
+            // blocks of 16 unknowns: the 16 x 16 diagonal block is solved by
+            // lanes 0..15 of wavefront 0 (SGPR broadcasts), the rest of the
+            // vector is updated by all threads.
             const int nb = (n + 15) / 16;
-            for (int blk = 0; blk < nb; ++blk) {
+            for (int blk = 0; blk < (aug ? 0 : nb); ++blk) {
                 const int i0 = 16 * blk, i1 = i0 + 16 < n ? i0 + 16 : n;
                 if (wave == 0) {
                     const int col = i0 + (lane & 15);
@@ -402,14 +410,23 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
             for (int blk = nb - 1; blk >= 0; --blk) {
                 const int i0 = 16 * blk, i1 = i0 + 16 < n ? i0 + 16 : n;
                 if (wave == 0) {
+                    // lane t owns x_i0+t; its row of the block and 1/U_tt sit in
+                    // registers, so a step is one multiply, broadcast and FMA
                     const int row = i0 + (lane & 15);
-                    double x = (lane < 16 && row < i1) ? dv[row] : 0.0;
-                    for (int i = i1 - 1; i >= i0; --i) {
-                        const double xi = lane_bcast(x, i - i0) * rdiag[i];
-                        if (lane < 16 && row < i) x -= H[bpidx(row, i)] * xi;
-                        if (lane == i - i0) x = xi;
+                    const bool own = lane < 16 && row < i1;
+                    double x = own ? dv[row] : 0.0;
+                    const double rd = own ? rdiag[row] : 0.0;
+                    double uh[16];
+#pragma unroll
+                    for (int t = 0; t < 16; ++t)
+                        uh[t] = (own && i0 + t < i1 && row < i0 + t) ? H[bpidx(row, i0 + t)] : 0.0;
+#pragma unroll
+                    for (int t = 15; t >= 0; --t) {
+                        if (i0 + t >= i1) continue;  // uniform
+                        const double xt = lane_bcast(x * rd, t);
+                        x = lane == t ? xt : fma(-uh[t], xt, x);
                     }
-                    if (lane < 16 && row < i1) dv[row] = x;
+                    if (own) dv[row] = x;
                 }
                 __syncthreads();
                 for (int r = tid; r < i0; r += kBlockThreads) {
