@@ -329,7 +329,7 @@ class GpuClosure:
         return self._check(lib().uqk_load_tuning(self.h, json_path.encode()), "uqk_load_tuning")
 
     def set_kernel(self, kernel):
-        """KERNEL_AUTO, KERNEL_GENERIC or KERNEL_WAVE."""
+        """KERNEL_AUTO, KERNEL_GENERIC, KERNEL_WAVE or KERNEL_BLOCK."""
         self._check(lib().uqk_set_closure_kernel(self.h, kernel), "uqk_set_closure_kernel")
 
     def last_kernel(self):
