@@ -125,6 +125,7 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
         __syncthreads();
 
         double Lk[S];  // Lambda at this thread's point (tid < Q) of the last gradient
+        double uproj = 0.0;  // <U phiW> of this thread's dual (tid < n) at the last gradient
         // gradient at X into gout (state-major e = l*nt + i); returns ||g||_2
         auto gradient = [&](const double* X, double* gout) -> double {
             // phase: gradient
@@ -148,6 +149,7 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                 const int l = tid / nt, i = tid - l * nt;
                 double acc = 0.0;
                 for (int k = 0; k < Q; ++k) acc += Ub[l * Q + k] * phiw[k * M + i];
+                uproj = acc;
                 acc -= uu[i * S + l];
                 gout[tid] = acc;
                 sq = acc * acc;
@@ -515,6 +517,19 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
         }
         if (status < 0) status = UQK_STATUS_MAX_ITERATIONS;
         for (int t = tid; t < SM; t += kBlockThreads) lg[t] = lam[t];
+        if (a.uq_out) {
+            // the step's next two sweeps from the last gradient, which is at
+            // the returned lambda: uQ = U(lambda phi) (momentsolver.cpp:100)
+            // and u = <uQ phiW> over the cell's nTotal, zero above (:120-124)
+            double* uqg = a.uq_out + cell * (long)Q * S;
+            for (int t = tid; t < Q * S; t += kBlockThreads) {
+                const int k = t / S, sx = t - k * S;
+                uqg[t] = Ub[sx * Q + k];
+            }
+            double* uo = a.u_out + cell * SM;
+            if (tid < n) uo[(tid - (tid / nt) * nt) * S + tid / nt] = uproj;
+            for (int t = nt * S + tid; t < SM; t += kBlockThreads) uo[t] = 0.0;
+        }
         if (tid == 0) {
             if (a.status) a.status[cell] = status;
             if (a.iters) a.iters[cell] = iters;

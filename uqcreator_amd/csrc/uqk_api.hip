@@ -86,8 +86,9 @@ int check_mesh(uqk_ctx* ctx, const uqk_mesh* m) {
 }
 
 // Launch the Newton closure kernel; leaves the failed-cell count in d_counters[1].
-// When uq_out/u_out are given and the specialised kernel runs (not one-shot),
-// it also writes uQ and the reconstructed moments and *fused is set.
+// When uq_out/u_out are given and the specialised or the workgroup kernel runs
+// (not one-shot), it also writes uQ and the reconstructed moments and *fused
+// is set.
 int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, double* lam, int32_t* status,
                    int32_t* iters, const int* active = nullptr, double* uq_out = nullptr, double* u_out = nullptr,
                    bool* fused = nullptr) {
@@ -136,13 +137,15 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     const bool block_ok = uqk::newton_block_fits(ctx->S, ctx->M, ctx->Q, (int)ctx->max_lds);
     if (ctx->kernel == UQK_KERNEL_BLOCK || (ctx->kernel == UQK_KERNEL_AUTO && ctx->S * ctx->M >= 64 && block_ok)) {
         if (!block_ok) return fail(ctx, -3, "closure too large for the workgroup kernel");
-        a.uq_out = nullptr;
-        a.u_out = nullptr;
+        const bool fuse = uq_out && u_out && ctx->c.max_iter != 1;
+        a.uq_out = fuse ? uq_out : nullptr;
+        a.u_out = fuse ? u_out : nullptr;
         if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         UQK_HIP(ctx, uqk::newton_block_launch(ctx->kind, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream));
         if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         ctx->last_kernel = UQK_KERNEL_BLOCK;
         ctx->timed = ev;
+        if (fused) *fused = fuse;
         return 0;
     }
     ctx->last_kernel = UQK_KERNEL_GENERIC;
