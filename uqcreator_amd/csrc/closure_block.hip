@@ -95,6 +95,7 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
 
     // guided work queue over workgroups (as in the wave kernels, per block)
     unsigned long long w_fail = 0, w_newton = 0, w_grad = 0;
+    double w_dt = 1e10;  // min ComputeDt over this workgroup's cells (fused path)
     const long qsize = 1L << a.qbits;
     long qpos = 0, qend = 0;
     for (;;) {
@@ -529,6 +530,16 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
             double* uo = a.u_out + cell * SM;
             if (tid < n) uo[(tid - (tid / nt) * nt) * S + tid / nt] = uproj;
             for (int t = nt * S + tid; t < SM; t += kBlockThreads) uo[t] = 0.0;
+            if (a.dt_min) {  // momentsolver.cpp:127-133 for this cell
+                double m = 1e10;
+                if (tid < Q) {
+                    double uk[S];
+#pragma unroll
+                    for (int sx = 0; sx < S; ++sx) uk[sx] = Ub[sx * Q + tid];
+                    m = point_dt<S>(a.dtp, uk, a.dx_param[cell]);
+                }
+                w_dt = fmin(w_dt, wave_min(m));  // per wavefront; combined below
+            }
         }
         if (tid == 0) {
             if (a.status) a.status[cell] = status;
@@ -544,6 +555,8 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
         if (w_newton) atomicAdd(a.nfail + 1, w_newton);
         if (w_grad) atomicAdd(a.nfail + 2, w_grad);
     }
+    // positive doubles order as their bit patterns; one atomic per wavefront
+    if (a.dt_min && lane == 0 && w_dt < 1e10) atomicMin(a.dt_min, (unsigned long long)__double_as_longlong(w_dt));
 }
 
 size_t newton_block_lds(int S, int M, int Q) {

@@ -142,6 +142,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
     // go to different chunks.  Newton and failure counts are summed per wave
     // (wave-uniform, scalar) and added once at the end.
     unsigned long long w_fail = 0, w_newton = 0, w_grad = 0;
+    double w_dt = 1e10;  // min ComputeDt over this wave's cells (fused path)
     const long qsize = 1L << a.qbits;
     long qpos = 0, qend = 0;
     for (;;) {
@@ -552,6 +553,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                 uqg[t] = Ub[s * QP + k];
             }
             if (lane < N) a.u_out[cell * SM + ei * S + el] = uproj;
+            if (a.dt_min) {  // momentsolver.cpp:127-133 for this cell
+                double m = 1e10;
+                if (lane < Q) {
+                    double uk[S];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) uk[s] = Ub[s * QP + lane];
+                    m = point_dt<S>(a.dtp, uk, a.dx_param[cell]);
+                }
+                w_dt = fmin(w_dt, wave_min(m));
+            }
         }
         if (lane == 0) {
             if (a.status) a.status[cell] = status;
@@ -567,6 +578,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
         if (w_fail) atomicAdd(a.nfail, w_fail);
         if (w_newton) atomicAdd(a.nfail + 1, w_newton);
         if (w_grad) atomicAdd(a.nfail + 2, w_grad);
+        // positive doubles order as their bit patterns
+        if (a.dt_min && w_dt < 1e10) atomicMin(a.dt_min, (unsigned long long)__double_as_longlong(w_dt));
     }
 }
 

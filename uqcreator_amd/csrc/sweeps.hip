@@ -356,6 +356,11 @@ __global__ void fill_u64_kernel(unsigned long long* p, int n) {
     if (t < n) p[t] = 0ull;
 }
 
+__global__ void fill_f64_kernel(double* p, int n, double v) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -436,6 +441,11 @@ hipError_t launch_mean_var(int kind, const SweepArgs& a, hipStream_t st) {
     if (a.ncell <= 0) return hipSuccess;
     if (a.Q > 128) return hipErrorInvalidValue;  // two quadrature points per lane
     UQK_DISPATCH(kind, go_mean_var, a, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(double* p, int n, double v, hipStream_t st) {
+    hipLaunchKernelGGL(fill_f64_kernel, dim3((n + 63) / 64), dim3(64), 0, st, p, n, v);
     return hipGetLastError();
 }
 

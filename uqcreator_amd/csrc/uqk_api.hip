@@ -89,10 +89,14 @@ int check_mesh(uqk_ctx* ctx, const uqk_mesh* m) {
 // When uq_out/u_out are given and the specialised or the workgroup kernel runs
 // (not one-shot), it also writes uQ and the reconstructed moments and *fused
 // is set.
+// With dt_out as well (the step), the fused kernels also reduce
+// Problem::ComputeDt over the cells into *dt_out and *fused_dt is set.
 int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, double* lam, int32_t* status,
                    int32_t* iters, const int* active = nullptr, double* uq_out = nullptr, double* u_out = nullptr,
-                   bool* fused = nullptr) {
+                   bool* fused = nullptr, double* dt_out = nullptr, const uqk::ProblemParams* dtp = nullptr,
+                   const double* dx_param = nullptr, bool* fused_dt = nullptr) {
     if (fused) *fused = false;
+    if (fused_dt) *fused_dt = false;
     ctx->timed = false;
     UQK_HIP(ctx, uqk::launch_fill_u64(ctx->d_counters, 4, ctx->stream));  // [4..7] belong to the warm start
     if (n <= 0) return 0;
@@ -116,6 +120,15 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     a.nfail = ctx->d_counters + 1;
     a.prec = ctx->precision;
     a.active = active;
+    const bool want_dt = dt_out && dtp && dx_param && uq_out && u_out && ctx->c.max_iter != 1;
+    if (want_dt) {
+        // the start value of ComputeDt's MIN (momentsolver.cpp:127); a path
+        // that does not fuse leaves dt to the dt sweep, which overwrites it
+        UQK_HIP(ctx, uqk::launch_fill_f64(dt_out, 1, 1e10, ctx->stream));
+        a.dtp = *dtp;
+        a.dx_param = dx_param;
+        a.dt_min = reinterpret_cast<unsigned long long*>(dt_out);
+    }
     const bool ev = !ctx->capturing;
     if (ctx->kernel == UQK_KERNEL_AUTO || ctx->kernel == UQK_KERNEL_WAVE) {
         hipError_t e = hipSuccess;
@@ -130,6 +143,7 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
             ctx->last_kernel = UQK_KERNEL_WAVE;
             ctx->timed = ev;
             if (fused) *fused = fuse;
+            if (fused_dt) *fused_dt = want_dt;
             return 0;
         }
         if (ctx->kernel == UQK_KERNEL_WAVE) return fail(ctx, -3, "no specialised closure kernel for this shape");
@@ -146,11 +160,13 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         ctx->last_kernel = UQK_KERNEL_BLOCK;
         ctx->timed = ev;
         if (fused) *fused = fuse;
+        if (fused_dt) *fused_dt = want_dt;
         return 0;
     }
     ctx->last_kernel = UQK_KERNEL_GENERIC;
     a.uq_out = nullptr;
     a.u_out = nullptr;
+    a.dt_min = nullptr;
     a.wave_doubles = uqk::newton_fp64_wave_doubles(ctx->S, ctx->M, ctx->Q);
     const size_t wave_bytes = sizeof(double) * (size_t)a.wave_doubles;
     const size_t table_bytes = sizeof(double) * 2 * (size_t)ctx->Q * ctx->M;
@@ -196,12 +212,13 @@ int enqueue_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, in
     const int64_t n = mesh->n_cells;
     const bool refining = ctx->refine.nlev > 1;
     if (refining && !nt) return fail(ctx, -1, "refinement levels need ntotal_cell");
-    bool fused = false;
+    bool fused = false, fused_dt = false;
+    const uqk::ProblemParams pp = problem_params(prob);
     int rc = launch_closure(ctx, n, nt, u, lam, status, iters, active, refining ? nullptr : uq,
-                            refining ? nullptr : u_scratch, &fused);
+                            refining ? nullptr : u_scratch, &fused, ctx->d_scalars, &pp, mesh->dx_param, &fused_dt);
     if (rc) return rc;
     uqk::SweepArgs a = sweep_args(ctx);
-    a.p = problem_params(prob);
+    a.p = pp;
     set_mesh(a, mesh);
     a.ncell = n;
     a.nt_cell = nt;
@@ -221,7 +238,7 @@ int enqueue_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, in
     a.u_out = u_scratch;
     if (!fused) UQK_HIP(ctx, uqk::launch_reconstruct(a, ctx->stream));
     a.scalar_out = ctx->d_scalars;
-    UQK_HIP(ctx, uqk::launch_dt(ctx->kind, a, ctx->stream));
+    if (!fused_dt) UQK_HIP(ctx, uqk::launch_dt(ctx->kind, a, ctx->stream));
     a.u = u_scratch;
     a.u_out = u;
     a.dt_dev = ctx->d_scalars;

@@ -31,6 +31,12 @@ struct NewtonArgs {
     double* u_out;               // returned lambda and its moments <U phiW> (fused uQ + reconstruction)
     long waves;                  // resident waves of the launch (work-queue chunking), set by the launcher
     int qbits;                   // work queue of 2^qbits positions >= ncell, set by the launcher
+    // with uq_out, when dt_min is not null: also min over cells and points of
+    // Problem::ComputeDt(uQ, dx_param) as the bits of a positive double,
+    // combined with atomicMin (the caller sets *dt_min = 1e10 first)
+    ProblemParams dtp;
+    const double* dx_param;
+    unsigned long long* dt_min;
 };
 
 // Device-side state of a multi-step run replayed from a captured graph.
@@ -131,6 +137,7 @@ hipError_t launch_advance(int kind, const SweepArgs& a, hipStream_t st);
 hipError_t launch_residual(const SweepArgs& a, hipStream_t st);
 hipError_t launch_mean_var(int kind, const SweepArgs& a, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, int n, hipStream_t st);
+hipError_t launch_fill_f64(double* p, int n, double v, hipStream_t st);
 int sweep_partial_blocks(long ncell);
 hipError_t launch_step_begin(RunState* rs, double t_end, long long max_steps, hipStream_t st);
 hipError_t launch_step_end(RunState* rs, const double* scalars, const unsigned long long* counters, hipStream_t st);
