@@ -13,7 +13,7 @@ One step = one Newton closure solve of every cell from lambda_0 = (DS(u_0),
 Legs: plain FP64 closure; mixed closure (FP32 MFMA Hessian, FP64 Cholesky,
 gradient and accept test); mixed closure with the learned warm start (a
 network trained here, outside the timed region, on Newton-solved pairs of
-other cells; the prediction and the per-cell judge run inside the timed
+other cells; the prediction and the per-cell selection run inside the timed
 region).  `value` is the last leg run; every leg reports its throughput, its
 error against the known duals and the rel. L2 of the mean/variance fields
 against the FP64 leg.  Inputs are resident in HBM before the timed region.
@@ -178,7 +178,7 @@ def main():
     ap.add_argument("--kernel", choices=["auto", "generic", "wave"], default="auto")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: the closure is independent per cell and this build measures one "
+        print("bench.py runs on one GPU only: the closure is independent per cell and this benchmark measures one "
               "GPU's share; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 

@@ -2,7 +2,7 @@
  * mlp_oracle.cpp -- FP64 CPU reference of the learned warm start's network.
  *
  * TEST INFRASTRUCTURE ONLY (same rule as oracle.cpp): loaded by tests/ as the
- * checker of the device forward/backward kernels.  The network has no
+ * reference the device forward/backward kernels are checked against.  The network has no
  * counterpart in UQCreator; it follows the textbook definitions:
  *   z_l = W_l a_{l-1} + b_l,  a_l = GELU(z_l) (l < L),  y = z_L,
  *   GELU(z) = z/2 (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3)))   (Hendrycks and

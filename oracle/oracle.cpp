@@ -1,7 +1,7 @@
 /*
  * oracle.cpp -- FP64 CPU restatement of UQCreator's IPM moment-solver hot path.
  *
- * TEST INFRASTRUCTURE.  This library is the checker for the HIP product path
+ * TEST INFRASTRUCTURE.  This library is what the HIP product path is checked against
  * and the timed CPU baseline.  Nothing in uqcreator_amd/ links or calls it.
  *
  * It restates the reference dataflow routine by routine (citations are

@@ -2,7 +2,7 @@
  * oracle.h -- FP64 CPU restatement of UQCreator's IPM moment-solver hot path.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * cpu_baseline leg of bench.py may load this library, and only to check results against
  * (or the timed CPU baseline).  The product (uqcreator_amd/) never links it.
  *
  * Every routine names the reference file:line it restates (paths relative to

@@ -1,7 +1,7 @@
 """ctypes wrapper of the FP64 CPU oracle (oracle/oracle.cpp).
 
 TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline.
+cpu_baseline leg of bench.py, always to check results against or as the timed CPU baseline.
 The product package (uqcreator_amd) never imports this module.
 """
 import ctypes as C

@@ -144,7 +144,7 @@ def test_select_keeps_the_better_start(oracle):
 
 def test_trained_warm_start_cuts_newton_steps():
     """Train on Newton-solved pairs of one seed, test on cells of another: the
-    judged warm start must not change the solution and must save Newton steps.
+    selected warm start must not change the solution and must save Newton steps.
     With the reference's damping alpha = 0.5 (Euler1D) Newton converges
     linearly at rate 1/2, so a start F times closer saves log2(F) steps: the
     gain is a few percent at this training size."""

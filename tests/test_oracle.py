@@ -1,5 +1,5 @@
 """The FP64 CPU oracle pinned against library fixtures, closed forms and
-invariants (it is the checker of the GPU path, so it is checked first)."""
+invariants (the GPU path is checked against it, so it is checked first)."""
 import numpy as np
 import pytest
 

@@ -1,7 +1,7 @@
 // mlp.hip -- learned warm start for the Newton closure (no counterpart in
 // UQCreator): a fully connected network maps normalised moments to a dual
 // correction, trained on the GPU on (u, lambda) pairs that the Newton closure
-// itself produced, and judged per cell against the plain start.
+// itself produced, and compared per cell with the plain start.
 //
 // Network: K0 inputs -> four hidden layers of width H with GELU (tanh form)
 // -> linear head of NO outputs.  K0 = NO = S*M rounded up to 32 (zero
@@ -341,7 +341,7 @@ __global__ void features_kernel(FeatArgs a) {
 }
 
 // lambda = lambda_lin + (mu_t + sd_t y) rho0/D_s; NaN where the mean state is
-// not physical (the judge then keeps the plain start).
+// not physical (the selection then keeps the plain start).
 template <int KIND>
 __global__ void apply_kernel(long n, const double* u, const float* y, int M, uqk::ClosureParams cp, int NO,
                              const float* mu_t, const float* sd_t, double* lam) {
@@ -371,7 +371,7 @@ __global__ void apply_kernel(long n, const double* u, const float* y, int M, uqk
     }
 }
 
-// Judge: per cell (one wave), ||grad L||_2 at lambda and at lambda_alt
+// Selection: per cell (one wave), ||grad L||_2 at lambda and at lambda_alt
 // (closure.cpp:234-253 gradient, (:195-203) norm); the smaller one stays in
 // lambda.  Non-finite norms never win.  counter += cells that took lambda_alt.
 template <int KIND>
@@ -681,7 +681,7 @@ int enqueue_predict(uqk_mlp* n, long ncell, const double* u, double* lambda_out)
     return 0;
 }
 
-// Judge kernel only (no readback): capturable.  counters[4] = cells that took
+// Selection kernel only (no readback): capturable.  counters[4] = cells that took
 // lambda_alt.
 int enqueue_warm_select(uqk_ctx* ctx, long ncell, const double* u, double* lambda, const double* lambda_alt,
                         const int* active) {
@@ -702,7 +702,7 @@ int enqueue_warm_select(uqk_ctx* ctx, long ncell, const double* u, double* lambd
 }
 
 // Warm start of the attached network before a closure: predict into the
-// network's lambda_alt buffer, then judge against lambda.  Capturable.
+// network's lambda_alt buffer, then select against lambda.  Capturable.
 int enqueue_warm_start(uqk_ctx* ctx, long ncell, const double* u, double* lambda, const int* active) {
     uqk_mlp* n = ctx->warm;
     if (!n) return 0;
