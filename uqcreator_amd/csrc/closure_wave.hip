@@ -65,6 +65,7 @@ struct WaveShape {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double d2 __attribute__((ext_vector_type(2)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 // PREC 0: FP64 Hessian on the vector units.  PREC 1: Hessian on MFMA
 // (v_mfma_f32_16x16x4f32, FP32 operands and accumulation); gradient, residual,
@@ -74,6 +75,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
     using W = WaveShape<KIND, M, Q>;
     constexpr int S = W::S, N = W::N, P = W::P, QP = W::QP;
     static_assert(N + 1 <= 64 && Q <= 64, "wave kernel needs n < 64 and Q <= 64");
+    static_assert(M <= 16 && S <= 16, "the gradient's MFMA product takes one 16 x 16 tile");
     extern __shared__ double smem[];
     if (a.active && *a.active == 0) return;
     const int lane = threadIdx.x & 63;
@@ -192,11 +194,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
 #pragma unroll
             for (int s = 0; s < S; ++s) Lk[s] = L[s];
             wave_sync();
-            double acc = 0.0;
-            const double* ur = Ub + el * QP;
-            const double* wr = phiwT + ei * QP;
-#pragma unroll 7
-            for (int k = 0; k < Q; ++k) acc += ur[k] * wr[k];
+            // <U phiW> of all duals as one FP64 MFMA product (S x Q)(Q x M):
+            // A(s, k) = U_s(k), B(k, i) = phiW(k, i) in 16 x 16 x 4 steps
+            // (lane l: row/column l % 16, k = 4 ks + l / 16); D(s, i) is
+            // element e of lane 16 (s - 4e) + i for s = 4e + l / 16, and goes
+            // to lane s M + i through the solve buffer yb (idle here).
+            f64x4 pacc = {0.0, 0.0, 0.0, 0.0};
+            {
+                const int r16 = lane & 15, kq4 = lane >> 4;
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks) {
+                    const int k = 4 * ks + kq4;
+                    const double av = (r16 < S && k < Q) ? Ub[r16 * QP + k] : 0.0;
+                    const double bv = (r16 < M && k < Q) ? phiwT[r16 * QP + k] : 0.0;
+                    pacc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, pacc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int sr = 4 * e + kq4;  // f64 layout: element e at (4e + l/16, l%16)
+                    if (sr < S && r16 < M) yb[sr * M + r16] = pacc[e];
+                }
+            }
+            wave_sync();
+            const double acc = lane < N ? yb[lane] : 0.0;
             uproj = acc;
             double g = lane < N ? acc - ue : 0.0;
             gout = g;
