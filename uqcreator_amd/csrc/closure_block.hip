@@ -145,13 +145,50 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                 for (int s = 0; s < S; ++s) Ub[s * Q + k] = Uv[s];
             }
             __syncthreads();
+            // <U phiW>(s, i) = sum_k U_s(k) phiW(k, i) as FP64 MFMA tiles of the
+            // product (S x Q)(Q x nt): one 16-column tile per wavefront, the
+            // table loads four k-steps ahead; results land in gout, state-major
+            {
+                const int r16 = lane & 15, kq4 = lane >> 4;
+                const int tnt = (nt + 15) >> 4, ks_n = (Q + 3) >> 2;
+                for (int ti = wave; ti < tnt; ti += kBlockThreads / 64) {
+                    const int ic = 16 * ti + r16;
+                    constexpr int PF = 4;
+                    auto load = [&](int ks, double& av, double& bv) {
+                        const int k = 4 * ks + kq4;
+                        av = (r16 < S && k < Q) ? Ub[r16 * Q + k] : 0.0;
+                        bv = (ic < nt && k < Q) ? phiw[k * M + ic] : 0.0;
+                    };
+                    double av[PF], bv[PF];
+#pragma unroll
+                    for (int u = 0; u < PF; ++u) load(u, av[u], bv[u]);
+                    f64x4 pacc = {0.0, 0.0, 0.0, 0.0};
+                    for (int ks0 = 0; ks0 < ks_n; ks0 += PF) {
+                        double an[PF], bn[PF];
+#pragma unroll
+                        for (int u = 0; u < PF; ++u) load(ks0 + PF + u, an[u], bn[u]);
+#pragma unroll
+                        for (int u = 0; u < PF; ++u) pacc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], pacc, 0, 0, 0);
+#pragma unroll
+                        for (int u = 0; u < PF; ++u) {
+                            av[u] = an[u];
+                            bv[u] = bn[u];
+                        }
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int sr = 4 * e + kq4;  // element e at (4e + l/16, l%16)
+                        if (sr < S && ic < nt) gout[sr * nt + ic] = pacc[e];
+                    }
+                }
+            }
+            __syncthreads();
             double sq = 0.0;
             if (tid < n) {
                 const int l = tid / nt, i = tid - l * nt;
-                double acc = 0.0;
-                for (int k = 0; k < Q; ++k) acc += Ub[l * Q + k] * phiw[k * M + i];
-                uproj = acc;
-                acc -= uu[i * S + l];
+                const double proj = gout[tid];
+                uproj = proj;
+                const double acc = proj - uu[i * S + l];
                 gout[tid] = acc;
                 sq = acc * acc;
             }
