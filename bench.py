@@ -172,7 +172,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--no-warm-start", action="store_true")
     ap.add_argument("--train-cells", type=int, default=1 << 19)
-    ap.add_argument("--train-steps", type=int, default=38400)
+    ap.add_argument("--train-steps", type=int, default=76800)
     ap.add_argument("--cpu-sample", type=int, default=1 << 17)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", choices=["auto", "generic", "wave"], default="auto")

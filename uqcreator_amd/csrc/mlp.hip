@@ -156,13 +156,20 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
             }
 }
 
-// db[n] += sum over rows of x[row*ld + n] (x FP32 or BF16), 256 rows a block.
+// db[n] += sum over rows of x[row*ld + n] (x FP32 or BF16): a block of 256
+// threads covers kColsumRows rows of 256 columns, eight loads in flight per
+// thread, one atomic per column and block.
+constexpr int kColsumRows = 32;
+inline dim3 colsum_grid(long rows, int N) {
+    return dim3((unsigned)((rows + kColsumRows - 1) / kColsumRows), (unsigned)((N + 255) / 256));
+}
 template <class T>
 __global__ __launch_bounds__(256) void colsum_kernel(const T* x, int ld, int rows, int N, float* db) {
-    const int n = threadIdx.x;
+    const int n = blockIdx.y * 256 + threadIdx.x;
     if (n >= N) return;
-    const int r0 = blockIdx.x * 256, r1 = min(rows, r0 + 256);
+    const int r0 = blockIdx.x * kColsumRows, r1 = min(rows, r0 + kColsumRows);
     float s = 0.f;
+#pragma unroll 8
     for (int r = r0; r < r1; ++r) s += (float)x[(size_t)r * ld + n];
     atomicAdd(&db[n], s);
 }
@@ -595,10 +602,10 @@ int backward(uqk_mlp* n, long rows, int n_real) {
         a.k_split = ksplit;
         if (int rc = gemm(n, EPI_ATOMIC, true, true, a, splits)) return rc;
         if (l == kLayers - 1)
-            hipLaunchKernelGGL(colsum_kernel<float>, dim3(blocks(rows)), dim3(256), 0, st, n->dyf, n->NO, (int)rows, N,
+            hipLaunchKernelGGL(colsum_kernel<float>, colsum_grid(rows, N), dim3(256), 0, st, n->dyf, n->NO, (int)rows, N,
                                gb);
         else
-            hipLaunchKernelGGL(colsum_kernel<__bf16>, dim3(blocks(rows)), dim3(256), 0, st, dcur, ldcur, (int)rows, N,
+            hipLaunchKernelGGL(colsum_kernel<__bf16>, colsum_grid(rows, N), dim3(256), 0, st, dcur, ldcur, (int)rows, N,
                                gb);
         if (l == 0) break;
         // dz_{l-1} = (dz_l W_l) * gelu'(z_{l-1})
