@@ -46,3 +46,35 @@ def test_run_steps_stops_at_t_end():
     dev.run_device(0.01)
     assert dev.steps == ref.steps and dev.t == ref.t >= 0.01
     np.testing.assert_array_equal(dev.u.download(), ref.u.download())
+
+
+@pytest.mark.parametrize("kind", ["burgers", "euler2d", "shallow_water"])
+def test_fused_dt_equals_the_dt_sweep(kind):
+    """The closure kernels that write uQ also reduce ComputeDt over their cells
+    (momentsolver.cpp:127-133); the step's dt must equal, bit for bit, the dt
+    sweep (uqk_compute_dt) run afterwards on the uQ the step wrote -- for the
+    wavefront kernel (Burgers, Euler 2D) and the workgroup kernel (shallow
+    water, n = 105)."""
+    from uqcreator_amd import mesh as meshes
+    if kind == "burgers":
+        b, m, u0, lam0, cl, prob = _burgers()
+    elif kind == "euler2d":
+        b, m, u0 = workloads.euler2d_c3(64, 16)
+        lam0 = np.zeros_like(u0)
+        lam0[:, 0] = workloads.euler2d_ds(u0[:, 0])
+        cl = native.GpuClosure(native.EULER2D, b, epsilon=1e-6)
+        prob = native.problem(native.EULER2D_PROBLEM, flux=native.FLUX_LAX_FRIEDRICHS, cfl=0.4)
+    else:
+        b, m, u0 = workloads.sw2d_c5(16)
+        lam0 = np.zeros_like(u0)
+        h = u0[:, 0, 0]
+        lam0[:, 0, 0] = 9.81 * h - 0.01 / h  # DS of water at rest with the log barrier
+        cl = native.GpuClosure(native.SHALLOW_WATER2D_BARRIER, b, barrier=0.01, epsilon=1e-8, safe_line_search=1)
+        prob = native.problem(native.SHALLOW_WATER2D_PROBLEM, cfl=0.45, friction=0.01, friction_dim=2)
+    print("case: fused dt", kind, flush=True)
+    s = solver.GpuMomentSolver(cl, prob, m)
+    s.set_state(u0, lam0)
+    for _ in range(3):
+        dt, _ = s.step()
+        assert cl.last_kernel() in (native.KERNEL_WAVE, native.KERNEL_BLOCK)
+        assert dt == cl.compute_dt(prob, s.dmesh, s.uq)

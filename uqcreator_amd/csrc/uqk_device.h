@@ -77,11 +77,12 @@ __device__ __forceinline__ void closure_U(const ClosureParams& c, const double* 
         out[1] = -(L[1] / L[2]) * r;
         out[2] = ((L[1] * L[1] - 2.0 * L[2]) / (2.0 * L[2] * L[2])) * r;
     } else if constexpr (KIND == UQK_EULER2D) {
-        // eulerclosure2d.cpp:7-17 (gamma stored negated there)
+        // eulerclosure2d.cpp:7-17 (gamma stored negated there):
+        // exp(a / (1 + gm)) (-v4)^(1 / (1 + gm)) taken as one exponential,
+        // exp((a + log(-v4)) / (1 + gm)): equal up to rounding, no pow
         const double gm = -c.gamma;
         double v1 = L[0], v2 = L[1], v3 = L[2], v4 = L[3];
-        double e = exp(1.0 / (1.0 + gm) * ((v2 * v2 + v3 * v3 - 2.0 * v1 * v4 - 2.0 * v4 * gm) / (2.0 * v4))) *
-                   pow(-v4, 1.0 / (1.0 + gm));
+        double e = exp(((v2 * v2 + v3 * v3 - 2.0 * v1 * v4 - 2.0 * v4 * gm) / (2.0 * v4) + log(-v4)) / (1.0 + gm));
         out[0] = e;
         out[1] = -(v2 * e / v4);
         out[2] = -(v3 * e / v4);
@@ -151,8 +152,8 @@ __device__ __forceinline__ void closure_DU(const ClosureParams& c, const double*
         const double gm = -c.gamma;
         double v1 = L[0], v2 = L[1], v3 = L[2], v4 = L[3];
         double v22 = v2 * v2, v32 = v3 * v3, v42 = v4 * v4;
-        double e = exp(1.0 / (1.0 + gm) * ((v22 + v32 - 2.0 * v1 * v4 - 2.0 * v4 * gm) / (2.0 * v4))) *
-                   pow(-v4, 1.0 / (1.0 + gm)) / (1.0 + gm);
+        // the density factor as in closure_U (one exponential, no pow)
+        double e = exp(((v22 + v32 - 2.0 * v1 * v4 - 2.0 * v4 * gm) / (2.0 * v4) + log(-v4)) / (1.0 + gm)) / (1.0 + gm);
         double vt = e * (v22 + v32 + 2.0 * v4 * gm) / (2.0 * v42 * v4);
         y[0] = -e;                                                  // (0,0)
         y[1] = v2 * e / v4;                                         // (1,0)
