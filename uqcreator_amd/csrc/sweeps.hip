@@ -12,6 +12,8 @@
 
 namespace uqk {
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
 constexpr int kSweepThreads = 256;
 constexpr int kPartialBlocks = 8192;  // grid of the grid-stride sweeps and size of the partials
 
@@ -148,6 +150,12 @@ __global__ void __launch_bounds__(kSweepThreads) sum_finish_kernel(SweepArgs a) 
     if (threadIdx.x == 0) *a.scalar_out = m;
 }
 
+// LDS doubles per wavefront of the advance: E x Q point fluxes of S states,
+// at least Q summed fluxes and the M x S projected ones.
+__host__ __device__ inline int advance_wave_doubles(int S, int E, int Q, int M) {
+    return (E * Q > Q + M ? E * Q : Q + M) * S;
+}
+
 // One wavefront per cell.  Lanes over quadrature points sum the point fluxes of
 // all edges (the projection onto phiw is linear, so the edges share one
 // projection); lanes over (i, s) then project and update.  The residual
@@ -160,10 +168,12 @@ __global__ void __launch_bounds__(kSweepThreads) advance_kernel(SweepArgs a) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int M = a.M, Q = a.Q, SM = S * M, E = a.nedge, D = a.dim;
-    double* Gs = smem + wave * E * Q * S;  // point fluxes per (edge, point)
+    // per wave: point fluxes per (edge, point), later the projected fluxes
+    const int WD = advance_wave_doubles(S, E, Q, M);
+    double* Gs = smem + wave * WD;
     // the projection table PhiW in LDS when it fits (launcher's choice)
     const bool wlds = a.table_lds != 0;
-    double* wt = smem + (blockDim.x >> 6) * E * Q * S;
+    double* wt = smem + (blockDim.x >> 6) * WD;
     if (wlds) {
         for (int t = threadIdx.x; t < Q * M; t += blockDim.x) wt[t] = a.phiw[t];
         __syncthreads();
@@ -237,13 +247,37 @@ __global__ void __launch_bounds__(kSweepThreads) advance_kernel(SweepArgs a) {
             Gs[t] = sum;
         }
         wave_sync();
+        // projection rhs(s, i) = sum_k Gsum(s, k) phiW(k, i) as FP64 MFMA
+        // tiles of (S x Q)(Q x M) (lane l: row/column l % 16, k = 4 ks + l/16),
+        // into the edge-flux buffer past the summed fluxes
+        double* rhsb = Gs + Q * S;
+        {
+            const int r16 = lane & 15, kq4 = lane >> 4;
+            const int ks_n = (Q + 3) >> 2;
+            for (int ti = 0; 16 * ti < M; ++ti) {
+                const int ic = 16 * ti + r16;
+                f64x4 pacc = {0.0, 0.0, 0.0, 0.0};
+                for (int ks = 0; ks < ks_n; ++ks) {
+                    const int k = 4 * ks + kq4;
+                    const double av = (r16 < S && k < Q) ? Gs[k * S + r16] : 0.0;
+                    const double bv = (ic < M && k < Q) ? phiw[k * M + ic] : 0.0;
+                    pacc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, pacc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int sr = 4 * e + kq4;  // element e at (4e + l/16, l%16)
+                    if (sr < S && ic < M) rhsb[ic * S + sr] = pacc[e];
+                }
+            }
+        }
+        wave_sync();
         const double f = dt / a.area[cell];
         const bool fric = (S == 3 && a.p.problem == UQK_SHALLOW_WATER2D_PROBLEM && a.p.friction > 0.0);
         for (int e = lane; e < SM; e += 64) {
             int i = e / S, s = e - i * S;
             double rhs = 0.0, src = 0.0;
             if (i < nt) {
-                for (int k = 0; k < Q; ++k) rhs += Gs[k * S + s] * phiw[k * M + i];
+                rhs = rhsb[e];
                 if (fric && s > 0) {
                     for (int k = 0; k < Q; ++k) {
                         const double* q = uqj + k * S;
@@ -413,7 +447,7 @@ hipError_t launch_dt(int kind, const SweepArgs& a, hipStream_t st) {
 
 template <int KIND> static void go_advance(const SweepArgs& a, hipStream_t st) {
     constexpr int S = KindInfo<KIND>::S;
-    size_t shmem = sizeof(double) * (kSweepThreads / 64) * a.nedge * a.Q * S;
+    size_t shmem = sizeof(double) * (kSweepThreads / 64) * advance_wave_doubles(S, a.nedge, a.Q, a.M);
     SweepArgs b = a;
     b.table_lds = a.Q * a.M <= 2048;  // PhiW up to 16 KB goes to LDS
     if (b.table_lds) shmem += sizeof(double) * a.Q * a.M;
