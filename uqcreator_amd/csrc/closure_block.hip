@@ -11,8 +11,9 @@
 //   * Hessian (closure.cpp:255-271): each symmetric state block on MFMA tiles
 //     (FP64, or FP32 with FP32 accumulation for the mixed path), operands
 //     straight from the L2-resident tables;
-//   * Cholesky (dposv 'U', vectorspace.cpp:152-171) on register tiles spread
-//     cyclically over the workgroup, two pivots per barrier; the triangular
+//   * Cholesky (dposv 'U', vectorspace.cpp:152-171) on 16 x 16 register tiles
+//     dealt over the four wavefronts, four pivots per step with the panel and
+//     the trailing update as FP64 MFMAs, two barriers per step; the triangular
 //     solves in blocks of 16 unknowns (a 16-lane sequential solve, then a
 //     workgroup-wide update of the rest);
 //   * every branch of the Newton iteration depends on workgroup-uniform
@@ -31,9 +32,10 @@ namespace uqk {
 namespace {
 
 constexpr int kBlockThreads = 256;
-constexpr int BRA = 4;  // row-pair slots: 128 rows
-constexpr int BNB = 8;  // column slots: 128 columns
 constexpr int kBlockMaxN = 128;
+// register tile slots per wavefront: 8 x 8 tiles of 16 x 16 cover n <= 128,
+// 36 of them on or above the diagonal, dealt to 4 wavefronts
+constexpr int kSlots = 9;
 
 __device__ __forceinline__ int bpidx(int r, int c) { return c * (c + 1) / 2 + r; }  // r <= c
 
@@ -89,7 +91,8 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
     double* DUk = Ub + S * Q;
     double* rdiag = DUk + Q * P;
     double* red = rdiag + SM;  // [0..3] reduction, [4] factorisation verdict
-    double* rowbuf = red + 8;  // 2 x 256: rows j, j+1 of U, double buffered
+    double* wbuf = red + 8;    // 16: W of the current diagonal block
+    double* rowbuf = wbuf + 16;  // 512: the current four rows of U, 64 per tile column
     long* qshare = reinterpret_cast<long*>(rowbuf + 512);
     const ClosureParams& c = a.c;
 
@@ -302,121 +305,161 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
         // d <- H^{-1} g (U^T U = H, upper, packed); false if not SPD
         auto cholesky_solve = [&](const double* g) -> bool {
             // phase: factor
-            // H (upper) is spread over the workgroup in registers: thread
-            // 16p + q holds rows 32a + 2p + i (i = 0, 1) of columns 16b + q in
-            // T[a][i][b] -- row pairs cyclic over the 16 row groups, columns
-            // cyclic over the 16 column groups.  A step eliminates two pivots:
-            // the 16 owner threads of the two rows (one wavefront) factor the
-            // 2x2 diagonal block from SGPR broadcasts, scale the rows and
-            // publish them in LDS; after one barrier every thread applies the
-            // rank-2 update to its tiles.
-            const int p16 = tid >> 4, q16 = tid & 15;
-            // with a free column the right-hand side rides along as column n:
-            // its elimination is the forward substitution U^T y = g
-            const bool aug = n < 16 * BNB;
-            double T[BRA][2][BNB];
+            // [H | g] (upper; with a free column the right-hand side rides
+            // along as column n, so its elimination is the forward
+            // substitution U^T y = g) in 16 x 16 tiles of the FP64 MFMA
+            // accumulator layout: element e of lane l of tile (I, J), I <= J,
+            // is row 16 I + 4 e + l/16, column 16 J + l%16.  Tile t =
+            // J(J+1)/2 + I lives in wavefront t % 4, register slot t / 4.  A
+            // step eliminates the four pivots j0..j0+3 (element e0 of tile row
+            // I0 = j0/16):
+            //   1. the wavefront holding tile (I0, I0) factors the 4x4 diagonal
+            //      block from SGPR broadcasts, H_dd = U_dd^T U_dd, and
+            //      publishes W = U_dd^-T;
+            //   2. the holders of tiles (I0, J) form the four rows of U,
+            //      P_J = W H(j0..j0+3, 16J..16J+15), as one MFMA 16x16x4 each
+            //      (A = W, B = element e0 of the tile as it stands) and
+            //      publish them;
+            //   3. every trailing tile takes H(I1, I2) -= P_I1^T P_I2, one MFMA
+            //      each (A = -P_I1, B = P_I2, both read in their lane layout).
+            // Two barriers per four pivots.
+            const bool aug = n < kBlockMaxN;
+            const int NT = (n + (aug ? 1 : 0) + 15) >> 4;
+            const int ntile = NT * (NT + 1) / 2;
+            const int wv = __builtin_amdgcn_readfirstlane(wave);
+            const int r16 = lane & 15, g4 = lane >> 4;
+            int sI[kSlots], sJ[kSlots];
+            bool sv[kSlots];
 #pragma unroll
-            for (int ta = 0; ta < BRA; ++ta)
+            for (int sl = 0; sl < kSlots; ++sl) {
+                const int t = 4 * sl + wv;
+                int J = 0;
+                while ((J + 1) * (J + 2) / 2 <= t) ++J;
+                sJ[sl] = J;
+                sI[sl] = t - J * (J + 1) / 2;
+                sv[sl] = t < ntile;
+            }
+            f64x4 T[kSlots];
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+            for (int sl = 0; sl < kSlots; ++sl)
 #pragma unroll
-                    for (int tb = 0; tb < BNB; ++tb) {
-                        const int r = 32 * ta + 2 * p16 + i, cc = 16 * tb + q16;
-                        T[ta][i][tb] = (r <= cc && cc < n) ? H[bpidx(r, cc)] : (aug && cc == n && r < n) ? g[r] : 0.0;
-                    }
-            for (int j = 0; j < n; j += 2) {
-                const bool two = j + 1 < n;
-                const int pj = (j & 31) >> 1, aj = j >> 5;
-                double* urow = rowbuf + 256 * ((j >> 1) & 1);  // rows j, j+1 at [c], [128 + c]
-                if (p16 == pj) {
-                    // the owner wavefront: lanes 16 (pj % 4) + q hold rows j, j+1
-                    const int lbase = 16 * (pj & 3);
-                    double h00 = 0.0, h01 = 0.0, h11 = 0.0;
+                for (int e = 0; e < 4; ++e) {
+                    const int r = 16 * sI[sl] + 4 * e + g4, cc = 16 * sJ[sl] + r16;
+                    T[sl][e] = !sv[sl] ? 0.0
+                               : (r <= cc && cc < n) ? H[bpidx(r, cc)]
+                               : (aug && cc == n && r < n) ? g[r] : 0.0;
+                }
+            // A operand slot of this lane in W (row l%16 < 4, column l/16 <= row)
+            const int wslot = (r16 < 4 && g4 <= r16) ? r16 * (r16 + 1) / 2 + g4 : -1;
+            auto pick = [](const f64x4& v, int e) { return e == 0 ? v[0] : e == 1 ? v[1] : e == 2 ? v[2] : v[3]; };
+            for (int j0 = 0; j0 < n; j0 += 4) {
+                const int I0 = j0 >> 4, e0 = (j0 & 15) >> 2;
+                const int np = n - j0 < 4 ? n - j0 : 4;  // real pivots; the rest become identity rows
+                const int td = I0 * (I0 + 1) / 2 + I0;   // tile (I0, I0)
+                if (wv == (td & 3)) {
+                    double dcol = 0.0;
 #pragma unroll
-                    for (int ta = 0; ta < BRA; ++ta) {
-                        if (ta != aj) continue;
+                    for (int sl = 0; sl < kSlots; ++sl)
+                        if (4 * sl + wv == td) dcol = pick(T[sl], e0);
+                    double h[4][4];
 #pragma unroll
-                        for (int tb = 0; tb < BNB; ++tb) {
-                            if (tb == (j >> 4)) h00 = lane_bcast(T[ta][0][tb], lbase + (j & 15));
-                            if (two && tb == ((j + 1) >> 4)) {
-                                h01 = lane_bcast(T[ta][0][tb], lbase + ((j + 1) & 15));
-                                h11 = lane_bcast(T[ta][1][tb], lbase + ((j + 1) & 15));
-                            }
-                        }
-                    }
-                    // U_j+1,j+1^2 = det / h00, so 1/U_j+1,j+1 = sqrt(h00) / sqrt(det)
-                    // and both square roots start at once
-                    const double det = fma(h00, h11, -h01 * h01);
-                    bool ok = h00 > 0.0 && (!two || det > 0.0);
-                    const double r0 = ok ? block_rsqrt(h00) : 0.0;
-                    const double u01 = h01 * r0;
-                    const double r1 = (two && ok) ? block_rsqrt(det) * (h00 * r0) : 0.0;
-                    if (q16 == 0) {
-                        rdiag[j] = r0;
-                        if (two) rdiag[j + 1] = r1;
+                    for (int x = 0; x < 4; ++x)
+#pragma unroll
+                        for (int y = x; y < 4; ++y)
+                            h[x][y] = y < np ? lane_bcast(dcol, 16 * x + (j0 & 15) + y) : (x == y ? 1.0 : 0.0);
+                    // two pivots at a time: 1/U_11 = sqrt(h00) / sqrt(det)
+                    const double det01 = fma(h[0][0], h[1][1], -h[0][1] * h[0][1]);
+                    bool ok = h[0][0] > 0.0 && det01 > 0.0;
+                    const double r0 = block_rsqrt(h[0][0]);
+                    const double rd01 = block_rsqrt(det01);
+                    const double u01 = h[0][1] * r0, u02 = h[0][2] * r0, u03 = h[0][3] * r0;
+                    const double r1 = rd01 * (h[0][0] * r0);
+                    const double u12 = (h[1][2] - u01 * u02) * r1, u13 = (h[1][3] - u01 * u03) * r1;
+                    const double g22 = h[2][2] - u02 * u02 - u12 * u12;
+                    const double g23 = h[2][3] - u02 * u03 - u12 * u13;
+                    const double g33 = h[3][3] - u03 * u03 - u13 * u13;
+                    const double det23 = fma(g22, g33, -g23 * g23);
+                    ok = ok && g22 > 0.0 && det23 > 0.0;
+                    const double r2 = block_rsqrt(g22);
+                    const double rd23 = block_rsqrt(det23);
+                    const double u23 = g23 * r2;
+                    const double r3 = rd23 * (g22 * r2);
+                    if (lane == 0) {
+                        // W = U_dd^-T, lower triangular, packed by rows
+                        const double w1 = -r1 * u01 * r0;
+                        const double w4 = -r2 * u12 * r1;
+                        const double w3 = -r2 * (u02 * r0 + u12 * w1);
+                        const double w8 = -r3 * u23 * r2;
+                        const double w7 = -r3 * (u13 * r1 + u23 * w4);
+                        const double w6 = -r3 * (u03 * r0 + u13 * w1 + u23 * w3);
+                        wbuf[0] = r0;
+                        wbuf[1] = w1;
+                        wbuf[2] = r1;
+                        wbuf[3] = w3;
+                        wbuf[4] = w4;
+                        wbuf[5] = r2;
+                        wbuf[6] = w6;
+                        wbuf[7] = w7;
+                        wbuf[8] = w8;
+                        wbuf[9] = r3;
+                        rdiag[j0] = r0;
+                        if (np > 1) rdiag[j0 + 1] = r1;
+                        if (np > 2) rdiag[j0 + 2] = r2;
+                        if (np > 3) rdiag[j0 + 3] = r3;
                         red[4] = ok ? 0.0 : 1.0;
-                    }
-#pragma unroll
-                    for (int ta = 0; ta < BRA; ++ta) {
-                        if (ta != aj) continue;
-#pragma unroll
-                        for (int tb = 0; tb < BNB; ++tb) {
-                            const int cc = 16 * tb + q16;
-                            // the same formulas give the diagonal; columns left of
-                            // the pivots or past n get values nothing reads
-                            const double v0 = T[ta][0][tb] * r0;
-                            T[ta][0][tb] = v0;
-                            urow[cc] = v0;
-                            if (two) {
-                                const double v1 = (T[ta][1][tb] - u01 * v0) * r1;
-                                T[ta][1][tb] = v1;
-                                urow[128 + cc] = v1;
-                            }
-                        }
                     }
                 }
                 __syncthreads();
-                if (red[4] != 0.0) return false;  // uniform: every thread reads the owners' verdict
-                const int jl = two ? j + 1 : j;
-                double uc0[BNB], uc1[BNB];
+                if (red[4] != 0.0) return false;  // uniform: every thread reads the owner's verdict
+                const double wop = wslot >= 0 ? wbuf[wslot] : 0.0;
 #pragma unroll
-                for (int tb = 0; tb < BNB; ++tb) {
-                    const int cc = 16 * tb + q16;
-                    const bool live = cc > jl && cc <= n;  // column n: the right-hand side
-                    uc0[tb] = live ? urow[cc] : 0.0;
-                    uc1[tb] = (live && two) ? urow[128 + cc] : 0.0;
+                for (int sl = 0; sl < kSlots; ++sl) {
+                    if (!sv[sl] || sI[sl] != I0) continue;
+                    const f64x4 z = {0.0, 0.0, 0.0, 0.0};
+                    const f64x4 pr = __builtin_amdgcn_mfma_f64_16x16x4f64(wop, pick(T[sl], e0), z, 0, 0, 0);
+                    // zero left of the diagonal
+                    rowbuf[64 * sJ[sl] + lane] = 16 * sJ[sl] + r16 < j0 + g4 ? 0.0 : pr[0];
                 }
+                __syncthreads();
+                if (j0 + 4 < n) {
+                    // tiles holding rows past the pivots; the next diagonal
+                    // tile first, it gates the next step
+                    const int In = (j0 + 4) >> 4;
 #pragma unroll
-                for (int ta = 0; ta < BRA; ++ta) {
-                    if (32 * ta + 31 <= jl) continue;  // uniform: every row of the slot is done
+                    for (int pass = 0; pass < 2; ++pass)
 #pragma unroll
-                    for (int i = 0; i < 2; ++i) {
-                        const int r = 32 * ta + 2 * p16 + i;
-                        // only rows past the pivots must keep their values;
-                        // entries below the diagonal or past n are never read
-                        if (r <= jl) continue;
-                        const double ur0 = urow[r], ur1 = two ? urow[128 + r] : 0.0;
-#pragma unroll
-                        for (int tb = 0; tb < BNB; ++tb) {
-                            if (16 * tb + 15 < 32 * ta) continue;  // slot wholly below the diagonal
-                            T[ta][i][tb] -= ur0 * uc0[tb] + ur1 * uc1[tb];
+                        for (int sl = 0; sl < kSlots; ++sl) {
+                            const int I1 = sI[sl], I2 = sJ[sl];
+                            if (!sv[sl] || I1 < I0 || 16 * I1 + 15 < j0 + 4) continue;
+                            if ((pass == 0) != (I1 == In && I2 == In)) continue;
+                            const double av = -rowbuf[64 * I1 + lane], bv = rowbuf[64 * I2 + lane];
+                            T[sl] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, T[sl], 0, 0, 0);
                         }
-                    }
+                }
+                // the panel's rows of U into their tiles
+#pragma unroll
+                for (int sl = 0; sl < kSlots; ++sl) {
+                    if (!sv[sl] || sI[sl] != I0) continue;
+                    const double pv = rowbuf[64 * sJ[sl] + lane];
+                    T[sl][0] = e0 == 0 ? pv : T[sl][0];
+                    T[sl][1] = e0 == 1 ? pv : T[sl][1];
+                    T[sl][2] = e0 == 2 ? pv : T[sl][2];
+                    T[sl][3] = e0 == 3 ? pv : T[sl][3];
                 }
             }
             // U back to LDS (packed) for the triangular solves
 #pragma unroll
-            for (int ta = 0; ta < BRA; ++ta)
+            for (int sl = 0; sl < kSlots; ++sl) {
+                if (!sv[sl]) continue;
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int tb = 0; tb < BNB; ++tb) {
-                        const int r = 32 * ta + 2 * p16 + i, cc = 16 * tb + q16;
-                        if (r <= cc && cc < n)
-                            H[bpidx(r, cc)] = T[ta][i][tb];
-                        else if (aug && cc == n && r < n)
-                            dv[r] = T[ta][i][tb];
-                    }
+                for (int e = 0; e < 4; ++e) {
+                    const int r = 16 * sI[sl] + 4 * e + g4, cc = 16 * sJ[sl] + r16;
+                    if (r <= cc && cc < n)
+                        H[bpidx(r, cc)] = T[sl][e];
+                    else if (aug && cc == n && r < n)
+                        dv[r] = T[sl][e];
+                }
+            }
             if (!aug)
                 for (int e = tid; e < n; e += kBlockThreads) dv[e] = g[e];
             __syncthreads();
@@ -598,7 +641,7 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
 
 size_t newton_block_lds(int S, int M, int Q) {
     const int SM = S * M, P = S * (S + 1) / 2;
-    return sizeof(double) * ((size_t)SM * (SM + 1) / 2 + 6 * (size_t)SM + (size_t)S * Q + (size_t)Q * P + SM + 8 + 512 + 2);
+    return sizeof(double) * ((size_t)SM * (SM + 1) / 2 + 6 * (size_t)SM + (size_t)S * Q + (size_t)Q * P + SM + 8 + 16 + 512 + 2);
 }
 
 template <int KIND>
