@@ -351,22 +351,26 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                 }
             // A operand slot of this lane in W (row l%16 < 4, column l/16 <= row)
             const int wslot = (r16 < 4 && g4 <= r16) ? r16 * (r16 + 1) / 2 + g4 : -1;
-            auto pick = [](const f64x4& v, int e) { return e == 0 ? v[0] : e == 1 ? v[1] : e == 2 ? v[2] : v[3]; };
-            for (int j0 = 0; j0 < n; j0 += 4) {
-                const int I0 = j0 >> 4, e0 = (j0 & 15) >> 2;
+            // the four steps of a tile row are unrolled: e0 and the broadcast
+            // lanes are constants
+            for (int I0 = 0; 16 * I0 < n; ++I0)
+#pragma unroll
+            for (int e0 = 0; e0 < 4; ++e0) {
+                const int j0 = 16 * I0 + 4 * e0;
+                if (j0 >= n) break;
                 const int np = n - j0 < 4 ? n - j0 : 4;  // real pivots; the rest become identity rows
                 const int td = I0 * (I0 + 1) / 2 + I0;   // tile (I0, I0)
                 if (wv == (td & 3)) {
                     double dcol = 0.0;
 #pragma unroll
                     for (int sl = 0; sl < kSlots; ++sl)
-                        if (4 * sl + wv == td) dcol = pick(T[sl], e0);
+                        if (4 * sl + wv == td) dcol = T[sl][e0];
                     double h[4][4];
 #pragma unroll
                     for (int x = 0; x < 4; ++x)
 #pragma unroll
                         for (int y = x; y < 4; ++y)
-                            h[x][y] = y < np ? lane_bcast(dcol, 16 * x + (j0 & 15) + y) : (x == y ? 1.0 : 0.0);
+                            h[x][y] = y < np ? lane_bcast(dcol, 16 * x + 4 * e0 + y) : (x == y ? 1.0 : 0.0);
                     // two pivots at a time: 1/U_11 = sqrt(h00) / sqrt(det)
                     const double det01 = fma(h[0][0], h[1][1], -h[0][1] * h[0][1]);
                     bool ok = h[0][0] > 0.0 && det01 > 0.0;
@@ -416,35 +420,31 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                 for (int sl = 0; sl < kSlots; ++sl) {
                     if (!sv[sl] || sI[sl] != I0) continue;
                     const f64x4 z = {0.0, 0.0, 0.0, 0.0};
-                    const f64x4 pr = __builtin_amdgcn_mfma_f64_16x16x4f64(wop, pick(T[sl], e0), z, 0, 0, 0);
+                    const f64x4 pr = __builtin_amdgcn_mfma_f64_16x16x4f64(wop, T[sl][e0], z, 0, 0, 0);
                     // zero left of the diagonal
                     rowbuf[64 * sJ[sl] + lane] = 16 * sJ[sl] + r16 < j0 + g4 ? 0.0 : pr[0];
                 }
                 __syncthreads();
                 if (j0 + 4 < n) {
-                    // tiles holding rows past the pivots; the next diagonal
-                    // tile first, it gates the next step
-                    const int In = (j0 + 4) >> 4;
+                    // trailing tiles, all slots in one basic block so the
+                    // scheduler overlaps the operand loads with the MFMAs: a
+                    // slot without rows past
+                    // the pivots gets zero operands (both, so stale LDS cannot
+                    // bring a NaN), which adds an exact zero; sI, sJ < 8 keep
+                    // the reads inside the buffer
 #pragma unroll
-                    for (int pass = 0; pass < 2; ++pass)
-#pragma unroll
-                        for (int sl = 0; sl < kSlots; ++sl) {
-                            const int I1 = sI[sl], I2 = sJ[sl];
-                            if (!sv[sl] || I1 < I0 || 16 * I1 + 15 < j0 + 4) continue;
-                            if ((pass == 0) != (I1 == In && I2 == In)) continue;
-                            const double av = -rowbuf[64 * I1 + lane], bv = rowbuf[64 * I2 + lane];
-                            T[sl] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, T[sl], 0, 0, 0);
-                        }
+                    for (int sl = 0; sl < kSlots; ++sl) {
+                        const int I1 = sI[sl], I2 = sJ[sl];
+                        const bool act = sv[sl] && I1 >= I0 && 16 * I1 + 15 >= j0 + 4;
+                        const double ra = rowbuf[64 * I1 + lane], rb = rowbuf[64 * I2 + lane];
+                        T[sl] = __builtin_amdgcn_mfma_f64_16x16x4f64(act ? -ra : 0.0, act ? rb : 0.0, T[sl], 0, 0, 0);
+                    }
                 }
                 // the panel's rows of U into their tiles
 #pragma unroll
                 for (int sl = 0; sl < kSlots; ++sl) {
                     if (!sv[sl] || sI[sl] != I0) continue;
-                    const double pv = rowbuf[64 * sJ[sl] + lane];
-                    T[sl][0] = e0 == 0 ? pv : T[sl][0];
-                    T[sl][1] = e0 == 1 ? pv : T[sl][1];
-                    T[sl][2] = e0 == 2 ? pv : T[sl][2];
-                    T[sl][3] = e0 == 3 ? pv : T[sl][3];
+                    T[sl][e0] = rowbuf[64 * sJ[sl] + lane];
                 }
             }
             // U back to LDS (packed) for the triangular solves
