@@ -13,7 +13,7 @@
 //     straight from the L2-resident tables;
 //   * Cholesky (dposv 'U', vectorspace.cpp:152-171) on 16 x 16 register tiles
 //     dealt over the four wavefronts, four pivots per step with the panel and
-//     the trailing update as FP64 MFMAs, two barriers per step; the triangular
+//     the trailing update as FP64 MFMAs, one barrier per step; the triangular
 //     solves in blocks of 16 unknowns (a 16-lane sequential solve, then a
 //     workgroup-wide update of the rest);
 //   * every branch of the Newton iteration depends on workgroup-uniform
@@ -90,10 +90,10 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
     double* Ub = dv + SM;
     double* DUk = Ub + S * Q;
     double* rdiag = DUk + Q * P;
-    double* red = rdiag + SM;  // [0..3] reduction, [4] factorisation verdict
-    double* wbuf = red + 8;    // 16: W of the current diagonal block
-    double* rowbuf = wbuf + 16;  // 512: the current four rows of U, 64 per tile column
-    long* qshare = reinterpret_cast<long*>(rowbuf + 512);
+    double* red = rdiag + SM;  // [0..3] reduction
+    double* dbuf = red + 8;      // 256: the current diagonal tile of the factorisation
+    double* rowbuf = dbuf + 256;  // 2 x 512: four rows of U, 64 per tile column, two interleaved buffers
+    long* qshare = reinterpret_cast<long*>(rowbuf + 1024);
     const ClosureParams& c = a.c;
 
     // guided work queue over workgroups (as in the wave kernels, per block)
@@ -313,16 +313,16 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
             // J(J+1)/2 + I lives in wavefront t % 4, register slot t / 4.  A
             // step eliminates the four pivots j0..j0+3 (element e0 of tile row
             // I0 = j0/16):
-            //   1. the wavefront holding tile (I0, I0) factors the 4x4 diagonal
-            //      block from SGPR broadcasts, H_dd = U_dd^T U_dd, and
-            //      publishes W = U_dd^-T;
+            //   1. every wavefront factors the 4x4 diagonal block from SGPR
+            //      broadcasts of its copy of tile (I0, I0), H_dd = U_dd^T U_dd,
+            //      and forms W = U_dd^-T;
             //   2. the holders of tiles (I0, J) form the four rows of U,
             //      P_J = W H(j0..j0+3, 16J..16J+15), as one MFMA 16x16x4 each
             //      (A = W, B = element e0 of the tile as it stands) and
             //      publish them;
             //   3. every trailing tile takes H(I1, I2) -= P_I1^T P_I2, one MFMA
             //      each (A = -P_I1, B = P_I2, both read in their lane layout).
-            // Two barriers per four pivots.
+            // One barrier per four pivots, one more per tile row.
             const bool aug = n < kBlockMaxN;
             const int NT = (n + (aug ? 1 : 0) + 15) >> 4;
             const int ntile = NT * (NT + 1) / 2;
@@ -351,29 +351,45 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                 }
             // A operand slot of this lane in W (row l%16 < 4, column l/16 <= row)
             const int wslot = (r16 < 4 && g4 <= r16) ? r16 * (r16 + 1) / 2 + g4 : -1;
-            // the four steps of a tile row are unrolled: e0 and the broadcast
-            // lanes are constants
-            for (int I0 = 0; 16 * I0 < n; ++I0)
-#pragma unroll
-            for (int e0 = 0; e0 < 4; ++e0) {
-                const int j0 = 16 * I0 + 4 * e0;
-                if (j0 >= n) break;
-                const int np = n - j0 < 4 ? n - j0 : 4;  // real pivots; the rest become identity rows
-                const int td = I0 * (I0 + 1) / 2 + I0;   // tile (I0, I0)
+            // the not-SPD verdict is the same in every wavefront (same
+            // values); a failed factorisation runs to the end on NaNs and
+            // reports false there
+            bool spd = true;
+            for (int I0 = 0; 16 * I0 < n; ++I0) {
+                // every wavefront copies the diagonal tile (I0, I0) and keeps
+                // its copy current with the MFMA the holder applies, so each
+                // factors the 4x4 diagonal blocks itself (same values, same
+                // verdict) and no barrier separates them from the panel
+                const int td = I0 * (I0 + 1) / 2 + I0;
                 if (wv == (td & 3)) {
-                    double dcol = 0.0;
 #pragma unroll
                     for (int sl = 0; sl < kSlots; ++sl)
-                        if (4 * sl + wv == td) dcol = T[sl][e0];
+                        if (4 * sl + wv == td)
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) dbuf[64 * e + lane] = T[sl][e];
+                }
+                __syncthreads();
+                f64x4 Dg;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) Dg[e] = dbuf[64 * e + lane];
+                // the four steps of the tile row, unrolled: e0 and the broadcast
+                // lanes are constants; the panel buffer alternates per step
+#pragma unroll
+                for (int e0 = 0; e0 < 4; ++e0) {
+                    const int j0 = 16 * I0 + 4 * e0;
+                    if (j0 >= n) break;
+                    const int np = n - j0 < 4 ? n - j0 : 4;  // real pivots; the rest become identity rows
+                    // P_J of lane l at pb[2 (64 J + l)]: the two buffers interleave, so
+                    // both use the same addresses (the parity is an immediate offset)
+                    double* pb = rowbuf + (e0 & 1);
                     double h[4][4];
 #pragma unroll
                     for (int x = 0; x < 4; ++x)
 #pragma unroll
                         for (int y = x; y < 4; ++y)
-                            h[x][y] = y < np ? lane_bcast(dcol, 16 * x + 4 * e0 + y) : (x == y ? 1.0 : 0.0);
+                            h[x][y] = y < np ? lane_bcast(Dg[e0], 16 * x + 4 * e0 + y) : (x == y ? 1.0 : 0.0);
                     // two pivots at a time: 1/U_11 = sqrt(h00) / sqrt(det)
                     const double det01 = fma(h[0][0], h[1][1], -h[0][1] * h[0][1]);
-                    bool ok = h[0][0] > 0.0 && det01 > 0.0;
                     const double r0 = block_rsqrt(h[0][0]);
                     const double rd01 = block_rsqrt(det01);
                     const double u01 = h[0][1] * r0, u02 = h[0][2] * r0, u03 = h[0][3] * r0;
@@ -383,70 +399,69 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                     const double g23 = h[2][3] - u02 * u03 - u12 * u13;
                     const double g33 = h[3][3] - u03 * u03 - u13 * u13;
                     const double det23 = fma(g22, g33, -g23 * g23);
-                    ok = ok && g22 > 0.0 && det23 > 0.0;
+                    spd = spd && h[0][0] > 0.0 && det01 > 0.0 && g22 > 0.0 && det23 > 0.0;
                     const double r2 = block_rsqrt(g22);
                     const double rd23 = block_rsqrt(det23);
                     const double u23 = g23 * r2;
                     const double r3 = rd23 * (g22 * r2);
-                    if (lane == 0) {
-                        // W = U_dd^-T, lower triangular, packed by rows
-                        const double w1 = -r1 * u01 * r0;
-                        const double w4 = -r2 * u12 * r1;
-                        const double w3 = -r2 * (u02 * r0 + u12 * w1);
-                        const double w8 = -r3 * u23 * r2;
-                        const double w7 = -r3 * (u13 * r1 + u23 * w4);
-                        const double w6 = -r3 * (u03 * r0 + u13 * w1 + u23 * w3);
-                        wbuf[0] = r0;
-                        wbuf[1] = w1;
-                        wbuf[2] = r1;
-                        wbuf[3] = w3;
-                        wbuf[4] = w4;
-                        wbuf[5] = r2;
-                        wbuf[6] = w6;
-                        wbuf[7] = w7;
-                        wbuf[8] = w8;
-                        wbuf[9] = r3;
+                    // W = U_dd^-T, lower triangular, packed by rows
+                    double w[10];
+                    w[0] = r0;
+                    w[2] = r1;
+                    w[1] = -r1 * u01 * r0;
+                    w[5] = r2;
+                    w[4] = -r2 * u12 * r1;
+                    w[3] = -r2 * (u02 * r0 + u12 * w[1]);
+                    w[9] = r3;
+                    w[8] = -r3 * u23 * r2;
+                    w[7] = -r3 * (u13 * r1 + u23 * w[4]);
+                    w[6] = -r3 * (u03 * r0 + u13 * w[1] + u23 * w[3]);
+                    if (tid == 0) {
                         rdiag[j0] = r0;
                         if (np > 1) rdiag[j0 + 1] = r1;
                         if (np > 2) rdiag[j0 + 2] = r2;
                         if (np > 3) rdiag[j0 + 3] = r3;
-                        red[4] = ok ? 0.0 : 1.0;
                     }
-                }
-                __syncthreads();
-                if (red[4] != 0.0) return false;  // uniform: every thread reads the owner's verdict
-                const double wop = wslot >= 0 ? wbuf[wslot] : 0.0;
+                    double wop = 0.0;
 #pragma unroll
-                for (int sl = 0; sl < kSlots; ++sl) {
-                    if (!sv[sl] || sI[sl] != I0) continue;
-                    const f64x4 z = {0.0, 0.0, 0.0, 0.0};
-                    const f64x4 pr = __builtin_amdgcn_mfma_f64_16x16x4f64(wop, T[sl][e0], z, 0, 0, 0);
-                    // zero left of the diagonal
-                    rowbuf[64 * sJ[sl] + lane] = 16 * sJ[sl] + r16 < j0 + g4 ? 0.0 : pr[0];
-                }
-                __syncthreads();
-                if (j0 + 4 < n) {
-                    // trailing tiles, all slots in one basic block so the
-                    // scheduler overlaps the operand loads with the MFMAs: a
-                    // slot without rows past
-                    // the pivots gets zero operands (both, so stale LDS cannot
-                    // bring a NaN), which adds an exact zero; sI, sJ < 8 keep
-                    // the reads inside the buffer
+                    for (int t = 0; t < 10; ++t) wop = wslot == t ? w[t] : wop;
 #pragma unroll
                     for (int sl = 0; sl < kSlots; ++sl) {
-                        const int I1 = sI[sl], I2 = sJ[sl];
-                        const bool act = sv[sl] && I1 >= I0 && 16 * I1 + 15 >= j0 + 4;
-                        const double ra = rowbuf[64 * I1 + lane], rb = rowbuf[64 * I2 + lane];
-                        T[sl] = __builtin_amdgcn_mfma_f64_16x16x4f64(act ? -ra : 0.0, act ? rb : 0.0, T[sl], 0, 0, 0);
+                        if (!sv[sl] || sI[sl] != I0) continue;
+                        const f64x4 z = {0.0, 0.0, 0.0, 0.0};
+                        const f64x4 pr = __builtin_amdgcn_mfma_f64_16x16x4f64(wop, T[sl][e0], z, 0, 0, 0);
+                        // zero left of the diagonal
+                        pb[2 * (64 * sJ[sl] + lane)] = 16 * sJ[sl] + r16 < j0 + g4 ? 0.0 : pr[0];
+                    }
+                    __syncthreads();
+                    if (j0 + 4 < n) {
+                        // trailing tiles, all slots in one basic block so the
+                        // scheduler overlaps the operand loads with the MFMAs:
+                        // a slot without rows past the pivots gets zero
+                        // operands (both, so stale LDS cannot bring a NaN),
+                        // which adds an exact zero; sI, sJ < 8 keep the reads
+                        // inside the buffer
+#pragma unroll
+                        for (int sl = 0; sl < kSlots; ++sl) {
+                            const int I1 = sI[sl], I2 = sJ[sl];
+                            const bool act = sv[sl] && I1 >= I0 && 16 * I1 + 15 >= j0 + 4;
+                            const double ra = pb[2 * (64 * I1 + lane)], rb = pb[2 * (64 * I2 + lane)];
+                            T[sl] = __builtin_amdgcn_mfma_f64_16x16x4f64(act ? -ra : 0.0, act ? rb : 0.0, T[sl], 0, 0, 0);
+                        }
+                        if (e0 < 3) {
+                            const double pd = pb[2 * (64 * I0 + lane)];
+                            Dg = __builtin_amdgcn_mfma_f64_16x16x4f64(-pd, pd, Dg, 0, 0, 0);
+                        }
+                    }
+                    // the panel's rows of U into their tiles
+#pragma unroll
+                    for (int sl = 0; sl < kSlots; ++sl) {
+                        if (!sv[sl] || sI[sl] != I0) continue;
+                        T[sl][e0] = pb[2 * (64 * sJ[sl] + lane)];
                     }
                 }
-                // the panel's rows of U into their tiles
-#pragma unroll
-                for (int sl = 0; sl < kSlots; ++sl) {
-                    if (!sv[sl] || sI[sl] != I0) continue;
-                    T[sl][e0] = rowbuf[64 * sJ[sl] + lane];
-                }
             }
+            if (!spd) return false;
             // U back to LDS (packed) for the triangular solves
 #pragma unroll
             for (int sl = 0; sl < kSlots; ++sl) {
@@ -641,7 +656,7 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
 
 size_t newton_block_lds(int S, int M, int Q) {
     const int SM = S * M, P = S * (S + 1) / 2;
-    return sizeof(double) * ((size_t)SM * (SM + 1) / 2 + 6 * (size_t)SM + (size_t)S * Q + (size_t)Q * P + SM + 8 + 16 + 512 + 2);
+    return sizeof(double) * ((size_t)SM * (SM + 1) / 2 + 6 * (size_t)SM + (size_t)S * Q + (size_t)Q * P + SM + 8 + 256 + 1024 + 2);
 }
 
 template <int KIND>
