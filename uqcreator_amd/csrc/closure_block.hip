@@ -79,7 +79,8 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
     const double* phi = a.phi;
     const double* phiw = a.phiw;
     // LDS: H (packed, SM(SM+1)/2) | lam | lamnew | u | g | gnew | d | Ub (S x Q) |
-    //      DU (Q x P) | rdiag (SM) | reduction (8) | queue (2)
+    //      DU (Q x P) | rdiag (SM) | reduction (8) | diagonal tile (256) |
+    //      rows of U (2 x 512) | queue (2)
     double* H = smem;
     double* lam = H + SM * (SM + 1) / 2;
     double* lamnew = lam + SM;
