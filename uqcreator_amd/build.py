@@ -34,6 +34,16 @@ HIP_FLAGS = [
 ]
 
 
+# extra flags per source file
+EXTRA_FLAGS = {
+    # the one-wavefront closure is a chain of dependent steps (pivot
+    # broadcasts, Newton steps); LLVM's iterative ILP scheduler orders it
+    # better than the default occupancy-driven one (about 3% more throughput
+    # for the mixed closure, the same for FP64)
+    "closure_wave.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
+}
+
+
 def sources():
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
     if os.path.isdir(HOST):
@@ -43,7 +53,7 @@ def sources():
 
 def _compile(src):
     obj = os.path.join(OUT, os.path.basename(src) + ".o")
-    cmd = [HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj]
+    cmd = [HIPCC] + HIP_FLAGS + EXTRA_FLAGS.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
     if src.endswith(".cpp"):
         cmd = [HIPCC, "-x", "c++"] + [f for f in HIP_FLAGS if not f.startswith("--offload")] + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
