@@ -381,6 +381,9 @@ __global__ void apply_kernel(long n, const double* u, const float* y, int M, uqk
 // Selection: per cell (one wave), ||grad L||_2 at lambda and at lambda_alt
 // (closure.cpp:234-253 gradient, (:195-203) norm); the smaller one stays in
 // lambda.  Non-finite norms never win.  counter += cells that took lambda_alt.
+// Persistent workgroups: phi and phiW are staged in LDS once per workgroup,
+// each wave walks cells with a grid stride and stages the candidate duals of
+// its cell in LDS (the sums run in the same order as a plain loop over k, i).
 template <int KIND>
 __global__ __launch_bounds__(256) void select_kernel(uqk::ClosureParams cp, long ncell, int M, int Q,
                                                      const double* phi, const double* phiw, const double* u,
@@ -388,42 +391,54 @@ __global__ __launch_bounds__(256) void select_kernel(uqk::ClosureParams cp, long
                                                      const int* active) {
     constexpr int S = uqk::KindInfo<KIND>::S;
     if (active && *active == 0) return;
-    extern __shared__ double uq_all[];
+    extern __shared__ double sel_lds[];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const long c = (long)blockIdx.x * 4 + w;
-    if (c >= ncell) return;
-    double* uq = uq_all + (size_t)w * Q * S;
     const int nf = M * S;
-    const double* uc = u + (size_t)c * nf;
-    double norm[2];
-    for (int cand = 0; cand < 2; ++cand) {
-        const double* l = (cand ? lam_alt : lam) + (size_t)c * nf;
-        for (int k = lane; k < Q; k += 64) {
-            double L[S], U[S];
-            for (int s = 0; s < S; ++s) L[s] = 0.0;
-            for (int i = 0; i < M; ++i) {
-                const double p = phi[(size_t)k * M + i];
-                for (int s = 0; s < S; ++s) L[s] += l[i * S + s] * p;
+    double* tphi = sel_lds;          // Q x M
+    double* tphiw = tphi + Q * M;    // Q x M
+    double* uq = tphiw + Q * M + (size_t)w * (Q * S + nf);
+    double* lw = uq + Q * S;
+    for (int t = threadIdx.x; t < Q * M; t += blockDim.x) {
+        tphi[t] = phi[t];
+        tphiw[t] = phiw[t];
+    }
+    __syncthreads();
+    unsigned long long took = 0;
+    for (long c = (long)blockIdx.x * 4 + w; c < ncell; c += (long)gridDim.x * 4) {
+        const double* uc = u + (size_t)c * nf;
+        double norm[2];
+        for (int cand = 0; cand < 2; ++cand) {
+            const double* l = (cand ? lam_alt : lam) + (size_t)c * nf;
+            for (int f = lane; f < nf; f += 64) lw[f] = l[f];
+            uqk::wave_sync();
+            for (int k = lane; k < Q; k += 64) {
+                double L[S], U[S];
+                for (int s = 0; s < S; ++s) L[s] = 0.0;
+                for (int i = 0; i < M; ++i) {
+                    const double p = tphi[k * M + i];
+                    for (int s = 0; s < S; ++s) L[s] += lw[i * S + s] * p;
+                }
+                uqk::closure_U<KIND>(cp, L, U);
+                for (int s = 0; s < S; ++s) uq[k * S + s] = U[s];
             }
-            uqk::closure_U<KIND>(cp, L, U);
-            for (int s = 0; s < S; ++s) uq[k * S + s] = U[s];
+            uqk::wave_sync();
+            double acc = 0.0;
+            for (int f = lane; f < nf; f += 64) {
+                const int i = f / S, s = f % S;
+                double gsum = 0.0;
+                for (int k = 0; k < Q; ++k) gsum += uq[k * S + s] * tphiw[k * M + i];
+                const double gf = gsum - uc[f];
+                acc += gf * gf;
+            }
+            norm[cand] = sqrt(uqk::wave_sum(acc));
+            uqk::wave_sync();
         }
-        uqk::wave_sync();
-        double acc = 0.0;
-        for (int f = lane; f < nf; f += 64) {
-            const int i = f / S, s = f % S;
-            double gsum = 0.0;
-            for (int k = 0; k < Q; ++k) gsum += uq[k * S + s] * phiw[(size_t)k * M + i];
-            const double gf = gsum - uc[f];
-            acc += gf * gf;
+        if (norm[1] < norm[0]) {
+            for (int f = lane; f < nf; f += 64) lam[(size_t)c * nf + f] = lam_alt[(size_t)c * nf + f];
+            ++took;
         }
-        norm[cand] = sqrt(uqk::wave_sum(acc));
-        uqk::wave_sync();
     }
-    if (norm[1] < norm[0]) {
-        for (int f = lane; f < nf; f += 64) lam[(size_t)c * nf + f] = lam_alt[(size_t)c * nf + f];
-        if (lane == 0) atomicAdd(counter, 1ull);
-    }
+    if (lane == 0 && took) atomicAdd(counter, took);
 }
 
 __global__ void rand_index_kernel(long* idx, long rows, long n, uint64_t seed, uint64_t step) {
@@ -696,8 +711,11 @@ int enqueue_warm_select(uqk_ctx* ctx, long ncell, const double* u, double* lambd
         return fail_ctx(ctx, -1, "warm start: supported for the Euler closures only");
     unsigned long long* cnt = ctx->d_counters + 4;
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, ctx->stream, cnt);
-    const size_t lds = sizeof(double) * 4 * ctx->Q * ctx->S;
-    const dim3 grid((unsigned)((ncell + 3) / 4));
+    // tables once per workgroup, then per wave the points' U values and the duals
+    const size_t lds = sizeof(double) * (2 * (size_t)ctx->Q * ctx->M + 4 * ((size_t)ctx->Q * ctx->S + (size_t)ctx->M * ctx->S));
+    if (lds > ctx->max_lds) return fail_ctx(ctx, -1, "warm start: selection tables exceed LDS");
+    const long want = (ncell + 3) / 4, cap = 8L * (ctx->num_cu > 0 ? ctx->num_cu : 1);
+    const dim3 grid((unsigned)(want < cap ? (want > 0 ? want : 1) : cap));
     if (ctx->kind == UQK_EULER1D)
         hipLaunchKernelGGL(select_kernel<UQK_EULER1D>, grid, dim3(256), lds, ctx->stream, ctx->c, ncell, ctx->M,
                            ctx->Q, ctx->d_phi, ctx->d_phiw, u, lambda, lambda_alt, cnt, active);
