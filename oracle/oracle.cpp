@@ -26,14 +26,22 @@
  *   orc_eval_uq             closure.cpp:217-230 + U(Matrix)             U of the closed form at each point
  *   orc_reconstruct         vectorspace.cpp:174-185, momentsolver.cpp   projection of polynomials is exact
  *                           :120-124
- *   orc_flux_point          burgers.cpp:21-38, euler2d.cpp:52-87,       consistency G(u,u)=F(u).n, conservation
- *                           shallowwater2d.cpp:21-53; LF form from      G(u,v,n)=-G(v,u,-n), upwind limits
- *                           julia/Tensorization/IPMSolver.jl:39-42
- *   orc_compute_dt          burgers.cpp:114-122, euler2d.cpp:114-134    closed forms
- *   orc_advance             expliciteuler.cpp:8-75                      conservation; uniform free stream stays
+ *   orc_flux_point          burgers.cpp:21-38, euler2d.cpp:52-87,       F(u).n written independently (rest state,
+ *                           shallowwater2d.cpp:21-53; LF form from      random states); HLL supersonic branches
+ *                           julia/Tensorization/IPMSolver.jl:39-42      return F(u).n / F(v).n exactly; subsonic
+ *                                                                       HLL and Rusanov closed forms; conservation
+ *   orc_compute_dt          burgers.cpp:114-122, euler2d.cpp:114-134    closed forms at hand states
+ *   orc_advance             expliciteuler.cpp:8-75                      free stream on a periodic sheared grid;
+ *                                                                       sum A u constant on a closed mesh; (u-unew)
+ *                                                                       /dt -> div F at first order; wall ghosts:
+ *                                                                       zero mass flux, pressure only; Dirichlet
+ *                                                                       frozen; friction projection closed form
  *   orc_run                 momentsolver.cpp:79-162                     Burgers exact solution (burgers.cpp:88-112)
- *   orc_mean_var            momentsolver.cpp:302-328                    closed form for lambda = (l0, 0, ...)
- *   orc_rel_error           momentsolver.cpp:687-749                    closed form
+ *   orc_mean_var            momentsolver.cpp:302-328                    200-point Gauss integral of the BB closed
+ *                                                                       form; constant dual gives Var = 0
+ *   orc_rel_error           momentsolver.cpp:687-749                    hand numbers (L1, L2, Linf, box mask)
+ *   (checks in tests/test_oracle_sweeps.py; each fails when a sign, a
+ *   pressure term, a scale or an operand order is planted wrong)
  *
  * Parity unpinned at bit level (no reference test covers them): LAPACK dposv's
  * last-ulp behaviour, boost legendre_p's last ulp, and the reference's exact
