@@ -148,8 +148,9 @@ def test_other_closures_match_oracle(oracle, kind, d, deg, q, extra, kernel):
     assert (np.abs(lam - lam_star) / scale).max() < 1e-7
 
 
-@pytest.mark.parametrize("kernel", [native.KERNEL_AUTO, native.KERNEL_GENERIC])
-def test_adaptive_orders_and_host_entry(oracle, kernel):
+@pytest.mark.parametrize("kernel,precision", [(native.KERNEL_AUTO, native.FP64), (native.KERNEL_GENERIC, native.FP64),
+                                              (native.KERNEL_AUTO, native.MIXED)])
+def test_adaptive_orders_and_host_entry(oracle, kernel, precision):
     # per-cell nTotal from nested total-degree levels (momentsolver.cpp:88);
     # AUTO takes the workgroup kernel (n up to 112)
     b = native.Basis(2, [1, 2, 3, 4, 5, 6], 7)
@@ -168,8 +169,8 @@ def test_adaptive_orders_and_host_entry(oracle, kernel):
     lam0 = np.zeros_like(lam_star)
     for j in range(n):
         lam0[j, 0] = oracle.DS(c, u[j, 0])
-    print("case: adaptive Euler2D M<=28 Q=49", flush=True)
-    cl = native.GpuClosure(native.EULER2D, b, epsilon=1e-9, safe_line_search=1)
+    print("case: adaptive Euler2D M<=28 Q=49 precision", precision, flush=True)
+    cl = native.GpuClosure(native.EULER2D, b, epsilon=1e-9, safe_line_search=1, precision=precision)
     cl.set_kernel(kernel)
     lam = lam0.copy()
     st, it = cl.solve_host(u, lam, nt_cell=nt)
@@ -266,3 +267,69 @@ def test_autotuned_launch_shape_gives_identical_duals(tmp_path):
     out2 = native.DeviceArray.from_host(lam0.download())
     assert cl2.solve(u, out2) == 0
     np.testing.assert_array_equal(out2.download(), ref.download())
+
+
+@pytest.mark.parametrize("case", ["sw2d_barrier_n105", "euler2d_orders_1_6"])
+def test_mixed_workgroup_kernel(oracle, case):
+    """The workgroup kernel's mixed path (Hessian tiles on FP32 MFMA with FP32
+    accumulation, FP64 factorisation, gradient and accept test) against the
+    FP64 oracle: the shallow-water barrier closure at n = 105 with 5^3 points
+    (the dam-break system) and Euler 2D with per-cell orders 1..6 on 7x7
+    points (nTotal 3..28, n up to 112).  Same statuses, duals within 1e-8 of
+    the oracle's relative to the cell's largest dual."""
+    rng = np.random.default_rng(31)
+    if case == "sw2d_barrier_n105":
+        b = native.Basis(3, [4], 5)
+        kind, okind, extra, n = native.SHALLOW_WATER2D_BARRIER, oracle.SW2D_BARRIER, dict(barrier=0.01), 96
+        c = oracle.closure(okind, eps=1e-8, safe=1, barrier=0.01)
+        lam_star = np.zeros((n, b.M, 3))
+        for j in range(n):
+            h, vx, vy = rng.uniform(1, 3), rng.uniform(-1, 1), rng.uniform(-1, 1)
+            lam_star[j, 0] = oracle.DS(c, np.array([h, h * vx, h * vy]))
+            lam_star[j, 1:] = rng.normal(0, 0.02, (b.M - 1, 3)) * np.abs(lam_star[j, 0])
+        nt = None
+        eps = 1e-8
+    else:
+        b = native.Basis(2, [1, 2, 3, 4, 5, 6], 7)
+        kind, okind, extra, n = native.EULER2D, oracle.EULER2D, {}, 384
+        c = oracle.closure(okind, eps=1e-9, safe=1)
+        ntl = b.ntotal_per_level
+        nt = ntl[rng.integers(0, len(ntl), n)].astype(np.int32)
+        nt[:6] = ntl  # every order present
+        lam_star = np.zeros((n, b.M, 4))
+        for j in range(n):
+            rho, vx, vy, p = rng.uniform(0.5, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.5, 2)
+            lam_star[j, 0] = oracle.DS(c, np.array([rho, rho * vx, rho * vy, p / 0.4 + 0.5 * rho * (vx * vx + vy * vy)]))
+            lam_star[j, 1:nt[j]] = rng.normal(0, 0.01, (nt[j] - 1, 4)) * np.abs(lam_star[j, 0])
+        eps = 1e-9
+    print("case: mixed workgroup kernel", case, "M", b.M, "Q", b.Q, flush=True)
+    uq = oracle.eval_uq(c, b.phi, lam_star, nt_cell=nt)
+    u = oracle.reconstruct(c.S, b.phiw, np.ascontiguousarray(uq), nt_cell=nt)
+    lam0 = np.zeros_like(lam_star)
+    for j in range(n):
+        lam0[j, 0] = oracle.DS(c, u[j, 0])
+    lam_o = lam0.copy()
+    st_o, it_o = oracle.solve_closure(c, b.phi, b.phiw, u, lam_o, nt_cell=nt)
+    assert (st_o == 0).all()
+    cl = native.GpuClosure(kind, b, epsilon=eps, safe_line_search=1, precision=native.MIXED, **extra)
+    cl.set_kernel(native.KERNEL_BLOCK)
+    nfail, lam, st, it = _run_gpu(cl, u, lam0, nt=nt)
+    assert cl.last_kernel() == native.KERNEL_BLOCK
+    np.testing.assert_array_equal(st, st_o)
+    assert nfail == 0
+    scale = np.abs(lam_star).max(axis=1, keepdims=True)
+    err = (np.abs(lam - lam_o) / scale).max()
+    print("max rel |lambda - lambda_oracle| = %.2e, Newton steps GPU %.2f oracle %.2f" % (err, it.mean(), it_o.mean()),
+          flush=True)
+    assert err < 1e-8
+    assert (np.abs(lam - lam_star) / scale).max() < 1e-7
+    assert it.mean() <= it_o.mean() + 1.0
+    if nt is not None:
+        for j in range(n):
+            assert np.all(lam[j, nt[j]:] == 0.0)
+    # the FP32 Hessian must give the FP64 kernel's fixed point, too
+    cl64 = native.GpuClosure(kind, b, epsilon=eps, safe_line_search=1, **extra)
+    cl64.set_kernel(native.KERNEL_BLOCK)
+    _, lam64, st64, _ = _run_gpu(cl64, u, lam0, nt=nt)
+    assert (st64 == 0).all()
+    assert (np.abs(lam - lam64) / scale).max() < 1e-8

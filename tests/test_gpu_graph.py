@@ -78,3 +78,48 @@ def test_fused_dt_equals_the_dt_sweep(kind):
         dt, _ = s.step()
         assert cl.last_kernel() in (native.KERNEL_WAVE, native.KERNEL_BLOCK)
         assert dt == cl.compute_dt(prob, s.dmesh, s.uq)
+
+
+def test_graph_recaptured_when_the_step_changes():
+    """A captured step replays only while what it launches is unchanged:
+    attaching or detaching the warm-start network, or changing the launch
+    shape, must re-capture.  The same solver buffers are reused so the
+    argument key alone would match.  The network has zero weights, so it
+    predicts the linearised start (one closed-form Newton step from
+    (DS(u_0), 0, ...)) and saves Newton steps on the first time step."""
+    b, m, u0 = workloads.euler2d_c3(32, 8)
+    lam0 = np.zeros_like(u0)
+    lam0[:, 0] = workloads.euler2d_ds(u0[:, 0])
+    prob = native.problem(native.EULER2D_PROBLEM, flux=native.FLUX_LAX_FRIEDRICHS, cfl=0.4)
+    cl = native.GpuClosure(native.EULER2D, b, epsilon=1e-6)
+    net = native.WarmStartNet(cl, hidden=64, seed=1)
+    net.set_params(np.zeros(net.n_params, np.float32))
+    s = solver.GpuMomentSolver(cl, prob, m)
+
+    def run(graph, attached):
+        if attached:
+            net.attach()
+        else:
+            net.detach()
+        s.set_state(u0, lam0)
+        s.t, s.steps, s.newton_steps = 0.0, 0, 0
+        s.run_device(1e3, max_steps=2, use_graph=graph)
+        return s.lam.download(), s.u.download(), s.newton_steps
+
+    print("case: graph recapture on warm start / shape changes", flush=True)
+    a_host = run(False, True)
+    a_graph = run(True, True)
+    d_graph = run(True, False)
+    d_host = run(False, False)
+    for x, y in ((a_graph, a_host), (d_graph, d_host)):
+        np.testing.assert_array_equal(x[0], y[0])
+        np.testing.assert_array_equal(x[1], y[1])
+        assert x[2] == y[2]
+    print("Newton steps with the warm start", a_host[2], "without", d_host[2], flush=True)
+    assert a_host[2] < d_host[2]
+    a2 = run(True, True)
+    np.testing.assert_array_equal(a2[0], a_host[0])
+    cl.set_launch_shape(1, 1)
+    d2 = run(True, False)
+    np.testing.assert_array_equal(d2[0], d_host[0])
+    assert d2[2] == d_host[2]

@@ -204,3 +204,34 @@ def test_euler2d_naca_short_loop(oracle):
         err = _rel_l2(mv, mv_o, m.area)
         print("rel L2 E/Var", err, flush=True)
         assert err.max() < 1e-8
+
+
+@pytest.mark.parametrize("precision,tol", [(native.FP64, 1e-8), (native.MIXED, 1e-6)])
+def test_euler2d_c3_full_grid(oracle, precision, tol):
+    """The 2D Euler config at its full size: NACA0012 512 x 128 O-grid (65,536
+    cells), M = 10, Q = 25, Rusanov flux, CFL 0.4, Newton tolerance 1e-6 on
+    the SI-unit states, 40 steps of the device loop replayed from the
+    captured graph, against the oracle's loop: E/Var rel L2 <= 1e-8 (FP64)
+    and <= 1e-6 (mixed)."""
+    b, m, u0 = workloads.euler2d_c3(512, 128)
+    lam0 = np.zeros_like(u0)
+    lam0[:, 0] = workloads.euler2d_ds(u0[:, 0])
+    steps = 40
+    print("case: C3 full grid 512x128, %d steps, precision %d" % (steps, precision), flush=True)
+    prob = native.problem(native.EULER2D_PROBLEM, flux=native.FLUX_LAX_FRIEDRICHS, cfl=0.4)
+    cl = native.GpuClosure(native.EULER2D, b, epsilon=1e-6, precision=precision)
+    s = solver.GpuMomentSolver(cl, prob, m)
+    s.set_state(u0, lam0)
+    s.run_device(1e3, max_steps=steps)
+    assert s.steps == steps
+    mv = s.mean_var()
+    c = oracle.closure(oracle.EULER2D, eps=1e-6)
+    uo, lo = u0.copy(), lam0.copy()
+    res = oracle.run(c, oracle.problem(oracle.P_EULER2D, flux=oracle.FLUX_LF, cfl=0.4), _oracle_mesh(oracle, m),
+                     b.phi, b.phiw, b.xi, uo, lo, 1e3, steps)
+    assert res["rc"] == 0 and res["steps"] == steps
+    assert abs(s.t - res["t"]) <= 1e-10 * res["t"]
+    mv_o = oracle.mean_var(c, b.phi, b.phiw, lo)
+    err = _rel_l2(mv, mv_o, m.area)
+    print("C3 full grid rel L2 E/Var", err, "newton GPU", s.newton_steps, "oracle", res["newton_iters"], flush=True)
+    assert err.max() < tol

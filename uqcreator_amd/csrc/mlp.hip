@@ -41,6 +41,7 @@ constexpr int kLayers = 5;  // four hidden + head
 constexpr int kTile = 64;   // GEMM workgroup tile (rows and columns)
 constexpr int kKs = 32;     // GEMM K step (one MFMA 16x16x32)
 constexpr int kLdsStride = 40;  // bf16 per LDS tile row (32 + 8 pad)
+constexpr long kPredictChunk = 1L << 16;  // rows per inference pass of the warm start
 
 enum { EPI_GELU = 0, EPI_LINEAR = 1, EPI_DGELU = 2, EPI_ATOMIC = 3 };
 
@@ -681,7 +682,7 @@ namespace uqk {
 // features -> network -> duals for ncell cells, in chunks of 64K rows; enqueue only
 int enqueue_predict(uqk_mlp* n, long ncell, const double* u, double* lambda_out) {
     if (ncell == 0) return 0;
-    const long chunk = 1L << 16;
+    const long chunk = kPredictChunk;
     if (int rc = ensure_rows(n, std::min<long>(ncell, chunk))) return rc;
     hipStream_t st = n->ctx->stream;
     const float* mu_t = n->norm + 2 * n->K0;
@@ -728,9 +729,9 @@ int enqueue_warm_select(uqk_ctx* ctx, long ncell, const double* u, double* lambd
 
 // Warm start of the attached network before a closure: predict into the
 // network's lambda_alt buffer, then select against lambda.  Capturable.
-int enqueue_warm_start(uqk_ctx* ctx, long ncell, const double* u, double* lambda, const int* active) {
+int prepare_warm_start(uqk_ctx* ctx, long ncell) {
     uqk_mlp* n = ctx->warm;
-    if (!n) return 0;
+    if (!n || ncell <= 0) return 0;
     if (n->lam_alt_cells < ncell) {
         if (ctx->capturing) return fail_ctx(ctx, -1, "warm start: buffers must exist before graph capture");
         double* p = nullptr;
@@ -738,6 +739,13 @@ int enqueue_warm_start(uqk_ctx* ctx, long ncell, const double* u, double* lambda
         n->lam_alt = p;
         n->lam_alt_cells = ncell;
     }
+    return ensure_rows(n, std::min<long>(ncell, kPredictChunk));
+}
+
+int enqueue_warm_start(uqk_ctx* ctx, long ncell, const double* u, double* lambda, const int* active) {
+    uqk_mlp* n = ctx->warm;
+    if (!n) return 0;
+    if (int rc = prepare_warm_start(ctx, ncell)) return rc;
     if (int rc = enqueue_predict(n, ncell, u, n->lam_alt)) return rc;
     return enqueue_warm_select(ctx, ncell, u, lambda, n->lam_alt, active);
 }
@@ -749,6 +757,7 @@ extern "C" {
 int uqk_set_warm_start(uqk_ctx* ctx, uqk_mlp* net) {
     if (!ctx) return -1;
     if (net && net->ctx != ctx) return fail_ctx(ctx, -1, "uqk_set_warm_start: network belongs to another context");
+    if (ctx->warm != net) uqk::invalidate_graph(ctx);
     ctx->warm = net;
     return 0;
 }
@@ -820,7 +829,12 @@ int uqk_mlp_create(uqk_ctx* ctx, int32_t hidden, uint64_t seed, uqk_mlp** out) {
 
 int uqk_mlp_destroy(uqk_mlp* n) {
     if (!n) return 0;
-    if (n->ctx && n->ctx->warm == n) n->ctx->warm = nullptr;
+    if (n->ctx && n->ctx->warm == n) {
+        // a captured step would still launch on this network's buffers
+        uqk::invalidate_graph(n->ctx);
+        n->ctx->warm = nullptr;
+    }
+    if (n->ctx) (void)hipStreamSynchronize(n->ctx->stream);
     for (void* q : n->owned) (void)hipFree(q);
     delete n;
     return 0;

@@ -339,6 +339,7 @@ int uqk_destroy(uqk_ctx* ctx) {
     hipFree(ctx->d_scalars);
     if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
     hipFree(ctx->d_run);
+    if (ctx->h_run) hipHostFree(ctx->h_run);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
@@ -350,6 +351,7 @@ const char* uqk_last_error(const uqk_ctx* ctx) { return ctx ? ctx->err.c_str() :
 
 int uqk_set_stream(uqk_ctx* ctx, void* s) {
     if (!ctx) return -1;
+    uqk::invalidate_graph(ctx);
     ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
     return 0;
 }
@@ -357,6 +359,7 @@ int uqk_set_stream(uqk_ctx* ctx, void* s) {
 int uqk_set_closure_kernel(uqk_ctx* ctx, int32_t kernel) {
     if (!ctx) return -1;
     if (kernel < UQK_KERNEL_AUTO || kernel > UQK_KERNEL_BLOCK) return fail(ctx, -1, "unknown closure kernel");
+    if (ctx->kernel != kernel) uqk::invalidate_graph(ctx);
     ctx->kernel = kernel;
     return 0;
 }
@@ -447,7 +450,7 @@ int uqk_set_refinement(uqk_ctx* ctx, int32_t n_levels, const int32_t* ntotal_per
     if (!ctx) return -1;
     if (n_levels <= 1) {
         ctx->refine = uqk::RefineArgs{};
-        ctx->graph_key.clear();
+        uqk::invalidate_graph(ctx);
         return 0;
     }
     if (n_levels > uqk::kMaxLevels || !ntotal_per_level)
@@ -468,7 +471,7 @@ int uqk_set_refinement(uqk_ctx* ctx, int32_t n_levels, const int32_t* ntotal_per
     r.refine_above = refine_above;
     r.coarsen_below = coarsen_below;
     ctx->refine = r;
-    ctx->graph_key.clear();  // the captured step changes
+    uqk::invalidate_graph(ctx);  // the captured step changes
     return 0;
 }
 
@@ -478,7 +481,7 @@ int uqk_set_refinement_cap(uqk_ctx* ctx, int32_t max_level) {
     if (max_level < 0 || max_level >= ctx->refine.nlev) return fail(ctx, -1, "refinement cap outside the levels");
     if (ctx->refine.max_level != max_level) {
         ctx->refine.max_level = max_level;
-        ctx->graph_key.clear();  // the captured step changes
+        uqk::invalidate_graph(ctx);  // the captured step changes
     }
     return 0;
 }
@@ -566,11 +569,14 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
     if (int rc = check_mesh(ctx, mesh)) return rc;
     UQK_HIP(ctx, hipSetDevice(ctx->device));
     if (!ctx->d_run) UQK_HIP(ctx, hipMalloc(&ctx->d_run, sizeof(uqk::RunState)));
-    uqk::RunState init{};
-    init.t = t0;
-    init.active = 1;
+    if (!ctx->h_run) UQK_HIP(ctx, hipHostMalloc((void**)&ctx->h_run, sizeof(uqk::RunState), hipHostMallocDefault));
+    // the warm start's buffers must exist before the step is captured
+    if (int rc = uqk::prepare_warm_start(ctx, mesh->n_cells)) return rc;
     UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    UQK_HIP(ctx, hipMemcpy(ctx->d_run, &init, sizeof(init), hipMemcpyHostToDevice));
+    *ctx->h_run = uqk::RunState{};
+    ctx->h_run->t = t0;
+    ctx->h_run->active = 1;
+    UQK_HIP(ctx, hipMemcpy(ctx->d_run, ctx->h_run, sizeof(uqk::RunState), hipMemcpyHostToDevice));
     const int* active = &ctx->d_run->active;
     auto enqueue = [&]() -> int {
         UQK_HIP(ctx, uqk::launch_step_begin(ctx->d_run, t_end, max_steps, ctx->stream));
@@ -580,14 +586,19 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
         return 0;
     };
     if (use_graph) {
-        char key[512];
-        snprintf(key, sizeof key, "%d %d %.17g %.17g %.17g %.17g %d|%lld %d %d %p %p %p %p %p %p %p|%p %p %p %p %p %p %p|%.17g %lld",
+        // the arguments and the context state the captured launches depend on
+        // (the setters of that state also drop the graph)
+        char key[640];
+        snprintf(key, sizeof key,
+                 "%d %d %.17g %.17g %.17g %.17g %d|%lld %d %d %p %p %p %p %p %p %p|%p %p %p %p %p %p %p|%.17g %lld|"
+                 "%d %d %d %p %p",
                  prob->problem, prob->flux, prob->cfl, prob->gamma, prob->gravity, prob->friction, prob->friction_dim,
                  (long long)mesh->n_cells, mesh->n_edges, mesh->dim, (const void*)mesh->neighbor,
                  (const void*)mesh->unit_normal, (const void*)mesh->normal, (const void*)mesh->area,
                  (const void*)mesh->dx_param, (const void*)mesh->bc, (const void*)mesh->boundary_normal,
                  (const void*)nt, (void*)u, (void*)lam, (void*)uq, (void*)u_scratch, (void*)status, (void*)iters,
-                 t_end, (long long)max_steps);
+                 t_end, (long long)max_steps, ctx->kernel, ctx->shape.waves, ctx->shape.blocks_per_cu,
+                 (void*)ctx->warm, (void*)ctx->stream);
         if (!ctx->graph_exec || ctx->graph_key != key) {
             if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
             ctx->graph_exec = nullptr;
@@ -608,8 +619,9 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
             ctx->graph_key = key;
         }
     }
-    // replay; every `chunk` steps read the run state back to stop early
-    uqk::RunState rs{};
+    // replay; every `chunk` steps read the run state back (into pinned host
+    // memory owned by the context) to stop early
+    uqk::RunState& rs = *ctx->h_run;
     int64_t launched = 0;
     const int64_t chunk = 64;
     const int64_t limit = max_steps < 0 ? INT64_MAX : max_steps;
@@ -624,7 +636,7 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
             }
         }
         launched += todo;
-        UQK_HIP(ctx, hipMemcpyAsync(&rs, ctx->d_run, sizeof(rs), hipMemcpyDeviceToHost, ctx->stream));
+        UQK_HIP(ctx, hipMemcpyAsync(ctx->h_run, ctx->d_run, sizeof(uqk::RunState), hipMemcpyDeviceToHost, ctx->stream));
         UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
         if (!rs.active || launched >= limit || rs.nfail) break;
     }
@@ -661,6 +673,7 @@ int uqk_set_launch_shape(uqk_ctx* ctx, int32_t waves, int32_t blocks_per_cu) {
     if (!ctx) return -1;
     if (waves != 0 && waves != 1 && waves != 2 && waves != 4) return fail(ctx, -1, "waves per workgroup: 0, 1, 2 or 4");
     if (blocks_per_cu < 0) return fail(ctx, -1, "workgroups per CU must be >= 0");
+    if (ctx->shape.waves != waves || ctx->shape.blocks_per_cu != blocks_per_cu) uqk::invalidate_graph(ctx);
     ctx->shape.waves = waves;
     ctx->shape.blocks_per_cu = blocks_per_cu;
     return 0;
@@ -715,6 +728,7 @@ int uqk_autotune(uqk_ctx* ctx, int64_t n, const double* u, const double* lambda,
     hipFree(scratch);
     ctx->kernel = kernel_keep;
     ctx->shape = rc ? keep : best;
+    uqk::invalidate_graph(ctx);
     if (rc) return rc;
     if (json_path) {
         FILE* f = std::fopen(json_path, "a");
@@ -742,6 +756,7 @@ int uqk_load_tuning(uqk_ctx* ctx, const char* json_path) {
                         "\"waves\": %d, \"blocks_per_cu\": %d",
                         &k, &S, &M, &Q, &p, &w, &b) == 7 &&
             k == ctx->kind && S == ctx->S && M == ctx->M && Q == ctx->Q && p == ctx->precision) {
+            if (ctx->shape.waves != w || ctx->shape.blocks_per_cu != b) uqk::invalidate_graph(ctx);
             ctx->shape.waves = w;  // the last matching line wins
             ctx->shape.blocks_per_cu = b;
             found = 1;

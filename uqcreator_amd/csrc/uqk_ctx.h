@@ -36,6 +36,7 @@ struct uqk_ctx {
     bool timed = false;  // ev0/ev1 bracket the last closure kernel
     bool capturing = false;  // enqueueing into a graph: no events, no syncs
     uqk::RunState* d_run = nullptr;
+    uqk::RunState* h_run = nullptr;  // pinned host copy of *d_run, read back by the replay loop
     hipGraphExec_t graph_exec = nullptr;
     std::string graph_key;
     size_t max_lds = 0;
@@ -47,6 +48,21 @@ struct uqk_ctx {
 };
 
 namespace uqk {
+// Drops the captured step graph.  Every setter that changes what the step
+// enqueues (kernel choice, launch shape, warm start, refinement, stream)
+// calls it, so the next uqk_run_steps captures afresh instead of replaying
+// launches that name a detached network, freed buffers or an old shape.
+inline void invalidate_graph(uqk_ctx* ctx) {
+    if (ctx->graph_exec) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipGraphExecDestroy(ctx->graph_exec);
+    }
+    ctx->graph_exec = nullptr;
+    ctx->graph_key.clear();
+}
+// Grows the attached network's buffers for a warm start over ncell cells
+// (outside graph capture, which cannot allocate).
+int prepare_warm_start(uqk_ctx* ctx, long ncell);
 // learned warm start (mlp.hip): predict with the network attached to ctx and
 // keep per cell the start with the smaller ||grad L||; enqueue only.
 int enqueue_warm_start(uqk_ctx* ctx, long ncell, const double* u, double* lambda, const int* active);
