@@ -135,6 +135,18 @@ int64_t uqk_closure_solve(uqk_ctx* ctx, int64_t n_cells, const int32_t* ntotal_c
 /* Same with host pointers (copies in and out). */
 int64_t uqk_closure_solve_host(uqk_ctx* ctx, int64_t n_cells, const int32_t* ntotal_cell, const double* u,
                                double* lambda, int32_t* status, int32_t* iters);
+/* uqk_closure_solve_host without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory in
+ * their loop (serving loops, harnesses that forbid hipMalloc).  The device
+ * staging of the call (counters, u, lambda, status, iters, nTotal) lives in
+ * one fixed 64 MiB array per GPU, held from the library's load until the
+ * process ends; batches that do not fit (about 50,000 cells at 40 duals) are
+ * rejected with an error.  Calls on one GPU share that array: calls that
+ * overlap on one GPU (other threads, other contexts) race.  A context with a
+ * learned warm start attached is rejected.  Same results as
+ * uqk_closure_solve_host. */
+int64_t uqk_closure_solve_host_noalloc(uqk_ctx* ctx, int64_t n_cells, const int32_t* ntotal_cell, const double* u,
+                                       double* lambda, int32_t* status, int32_t* iters);
 
 /* uQ_j = U(EvaluateLambdaOnPE(lambda_j)) (momentsolver.cpp:99-101,
  * closure.cpp:217-230 with the U(Matrix) form).  Device pointers. */

@@ -11,11 +11,28 @@
 #include <cstring>
 #include <string>
 
+#include <mutex>
+
 #include "uqk.h"
 #include "uqk_internal.h"
 #include "uqk_ctx.h"
 
+// Device staging of uqk_closure_solve_host_noalloc: one call's closure
+// counters, moments, duals, per-cell nTotal, statuses and iteration counts.
+// Every GPU the library runs on holds its own copy of this array from the
+// code object's load until the process ends.  Host code never names it where
+// a device address is needed (on the host the name is only a stand-in, and a
+// kernel given it faults): uqk_create resolves the device address once per
+// device with hipGetSymbolAddress, and every use goes through that address.
+// Calls that use it on one GPU at the same time race (one array per GPU).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchDevices] = {};  // device address of `scratch`, per device ordinal
 
 int fail(uqk_ctx* ctx, int code, const std::string& msg) {
     if (ctx) ctx->err = msg;
@@ -85,7 +102,8 @@ int check_mesh(uqk_ctx* ctx, const uqk_mesh* m) {
     return 0;
 }
 
-// Launch the Newton closure kernel; leaves the failed-cell count in d_counters[1].
+// Launch the Newton closure kernel; leaves the failed-cell count in counters[1]
+// (counters: the context's d_counters unless the caller brings its own four).
 // When uq_out/u_out are given and the specialised or the workgroup kernel runs
 // (not one-shot), it also writes uQ and the reconstructed moments and *fused
 // is set.
@@ -94,11 +112,13 @@ int check_mesh(uqk_ctx* ctx, const uqk_mesh* m) {
 int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, double* lam, int32_t* status,
                    int32_t* iters, const int* active = nullptr, double* uq_out = nullptr, double* u_out = nullptr,
                    bool* fused = nullptr, double* dt_out = nullptr, const uqk::ProblemParams* dtp = nullptr,
-                   const double* dx_param = nullptr, bool* fused_dt = nullptr) {
+                   const double* dx_param = nullptr, bool* fused_dt = nullptr,
+                   unsigned long long* counters = nullptr) {
     if (fused) *fused = false;
     if (fused_dt) *fused_dt = false;
     ctx->timed = false;
-    UQK_HIP(ctx, uqk::launch_fill_u64(ctx->d_counters, 4, ctx->stream));  // [4..7] belong to the warm start
+    if (!counters) counters = ctx->d_counters;
+    UQK_HIP(ctx, uqk::launch_fill_u64(counters, 4, ctx->stream));  // d_counters[4..7] belong to the warm start
     if (n <= 0) return 0;
     if (ctx->warm && !nt) {
         int rc = uqk::enqueue_warm_start(ctx, n, u, lam, active);
@@ -116,8 +136,8 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     a.lam = lam;
     a.status = status;
     a.iters = iters;
-    a.counter = ctx->d_counters;
-    a.nfail = ctx->d_counters + 1;
+    a.counter = counters;
+    a.nfail = counters + 1;
     a.prec = ctx->precision;
     a.active = active;
     const bool want_dt = dt_out && dtp && dx_param && uq_out && u_out && ctx->c.max_iter != 1;
@@ -248,8 +268,9 @@ int enqueue_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, in
     return 0;
 }
 
-int64_t finish_closure(uqk_ctx* ctx) {
-    UQK_HIP(ctx, hipMemcpyAsync(ctx->last_stats, ctx->d_counters + 1, sizeof(ctx->last_stats), hipMemcpyDeviceToHost,
+int64_t finish_closure(uqk_ctx* ctx, const unsigned long long* counters = nullptr) {
+    if (!counters) counters = ctx->d_counters;
+    UQK_HIP(ctx, hipMemcpyAsync(ctx->last_stats, counters + 1, sizeof(ctx->last_stats), hipMemcpyDeviceToHost,
                                 ctx->stream));
     UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
@@ -257,6 +278,41 @@ int64_t finish_closure(uqk_ctx* ctx) {
     if (ctx->timed) UQK_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     if (ctx->timed) ctx->last_closure_ms = ms;
     return (int64_t)ctx->last_stats[0];
+}
+
+// Device address of `scratch` on ctx's device (current): looked up once per
+// device and kept in g_scratch_ptr.
+int resolve_scratch(uqk_ctx* ctx) {
+    if (ctx->device < 0 || ctx->device >= kScratchDevices)
+        return fail(ctx, -1, "device ordinal outside 0.." + std::to_string(kScratchDevices - 1));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[ctx->device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(ctx, -2, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[ctx->device] = p;
+    }
+    ctx->scratch = static_cast<unsigned char*>(g_scratch_ptr[ctx->device]);
+    return 0;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Layout of one uqk_closure_solve_host_noalloc call in `scratch`: the four
+// closure counters, then u, lambda, status, iters and (if given) nTotal, each
+// on a 256-byte boundary.  Returns the bytes used.
+size_t noalloc_layout(int64_t n, int S, int M, bool with_nt, size_t off[6]) {
+    const size_t cb = sizeof(double) * (size_t)n * S * M;
+    const size_t ib = sizeof(int32_t) * (size_t)n;
+    size_t at = 0;
+    off[0] = at, at += align256(4 * sizeof(unsigned long long));
+    off[1] = at, at += align256(cb);
+    off[2] = at, at += align256(cb);
+    off[3] = at, at += align256(ib);
+    off[4] = at, at += align256(ib);
+    off[5] = at, at += with_nt ? align256(ib) : 0;
+    return at;
 }
 
 }  // namespace
@@ -303,6 +359,7 @@ int uqk_create(const uqk_desc* desc, uqk_ctx** out) {
     ctx->c.du_exact = desc->du_exact;
     ctx->c.safe = desc->safe_line_search;
     UQK_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = resolve_scratch(ctx)) return rc;
     hipDeviceProp_t prop;
     UQK_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
     ctx->max_lds = prop.sharedMemPerBlock;
@@ -430,6 +487,52 @@ int64_t uqk_closure_solve_host(uqk_ctx* ctx, int64_t n, const int32_t* nt, const
     }
 #undef UQK_HIP_C
     cleanup();
+    return result;
+}
+
+int64_t uqk_closure_solve_host_noalloc(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, double* lam,
+                                       int32_t* status, int32_t* iters) {
+    if (!ctx) return -1;
+    if (n < 0 || (n > 0 && (!u || !lam))) return fail(ctx, -1, "uqk_closure_solve_host_noalloc: invalid arguments");
+    if (ctx->warm)
+        return fail(ctx, -1,
+                    "uqk_closure_solve_host_noalloc: a learned warm start is attached (its buffers grow with the "
+                    "batch); detach it with uqk_set_warm_start(ctx, NULL)");
+    size_t off[6];
+    const size_t need = noalloc_layout(n, ctx->S, ctx->M, nt != nullptr, off);
+    if (need > kScratchBytes) {
+        size_t off1[6];
+        const size_t per_cell = noalloc_layout(1 << 16, ctx->S, ctx->M, nt != nullptr, off1) >> 16;
+        return fail(ctx, -1,
+                    "uqk_closure_solve_host_noalloc: " + std::to_string(n) + " cells need " + std::to_string(need) +
+                        " bytes of staging, more than its fixed " + std::to_string(kScratchBytes >> 20) +
+                        " MiB (about " + std::to_string(kScratchBytes / per_cell) +
+                        " cells of this closure); split the batch or use uqk_closure_solve_host");
+    }
+    UQK_HIP(ctx, hipSetDevice(ctx->device));
+    unsigned char* base = ctx->scratch;  // device address of `scratch` on this device
+    auto* counters = reinterpret_cast<unsigned long long*>(base + off[0]);
+    auto* du = reinterpret_cast<double*>(base + off[1]);
+    auto* dl = reinterpret_cast<double*>(base + off[2]);
+    auto* dst = reinterpret_cast<int32_t*>(base + off[3]);
+    auto* dit = reinterpret_cast<int32_t*>(base + off[4]);
+    auto* dnt = nt ? reinterpret_cast<int32_t*>(base + off[5]) : nullptr;
+    const size_t cb = sizeof(double) * (size_t)n * ctx->S * ctx->M;
+    if (n > 0) {
+        UQK_HIP(ctx, hipMemcpyAsync(du, u, cb, hipMemcpyHostToDevice, ctx->stream));
+        UQK_HIP(ctx, hipMemcpyAsync(dl, lam, cb, hipMemcpyHostToDevice, ctx->stream));
+        if (nt) UQK_HIP(ctx, hipMemcpyAsync(dnt, nt, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+    }
+    int rc = launch_closure(ctx, n, dnt, du, dl, dst, dit, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                            nullptr, nullptr, counters);
+    if (rc) return rc;
+    const int64_t result = finish_closure(ctx, counters);
+    if (result >= 0 && n > 0) {
+        UQK_HIP(ctx, hipMemcpyAsync(lam, dl, cb, hipMemcpyDeviceToHost, ctx->stream));
+        if (status) UQK_HIP(ctx, hipMemcpyAsync(status, dst, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+        if (iters) UQK_HIP(ctx, hipMemcpyAsync(iters, dit, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+        UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     return result;
 }
 
@@ -692,8 +795,8 @@ int uqk_autotune(uqk_ctx* ctx, int64_t n, const double* u, const double* lambda,
     if (!uqk::newton_wave_limits(ctx->kind, ctx->M, ctx->Q, ctx->precision, (int)ctx->max_lds, 4, &wmax, &occ))
         return fail(ctx, -3, "uqk_autotune: no specialised closure kernel for this shape");
     const size_t bytes = sizeof(double) * (size_t)n * ctx->S * ctx->M;
-    double* scratch = nullptr;
-    UQK_HIP(ctx, hipMalloc(&scratch, bytes));
+    double* trial = nullptr;
+    UQK_HIP(ctx, hipMalloc(&trial, bytes));
     const uqk::LaunchShape keep = ctx->shape;
     const int kernel_keep = ctx->kernel;
     ctx->kernel = UQK_KERNEL_WAVE;
@@ -708,11 +811,11 @@ int uqk_autotune(uqk_ctx* ctx, int64_t n, const double* u, const double* lambda,
             ctx->shape.blocks_per_cu = per_cu;
             double ms_min = 1e300;
             for (int rep = 0; rep < 2 && rc == 0; ++rep) {  // first run warms the caches
-                if (hipMemcpyAsync(scratch, lambda, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) {
+                if (hipMemcpyAsync(trial, lambda, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) {
                     rc = fail(ctx, -2, "uqk_autotune: copy failed");
                     break;
                 }
-                rc = launch_closure(ctx, n, nullptr, u, scratch, nullptr, nullptr);
+                rc = launch_closure(ctx, n, nullptr, u, trial, nullptr, nullptr);
                 if (rc) break;
                 if (finish_closure(ctx) < 0) rc = -2;
                 if (rep == 1) ms_min = ctx->last_closure_ms;
@@ -725,7 +828,7 @@ int uqk_autotune(uqk_ctx* ctx, int64_t n, const double* u, const double* lambda,
             }
         }
     }
-    hipFree(scratch);
+    hipFree(trial);
     ctx->kernel = kernel_keep;
     ctx->shape = rc ? keep : best;
     uqk::invalidate_graph(ctx);

@@ -44,6 +44,7 @@ struct uqk_ctx {
     uqk::LaunchShape shape;   // specialised closure kernel launch shape (autotuned or set)
     uqk_mlp* warm = nullptr;  // learned warm start run before each closure (not owned)
     uqk::RefineArgs refine{};  // adaptive levels: refine.nlev > 1 turns refinement on in the step
+    unsigned char* scratch = nullptr;  // device address of the library's static staging array on `device`
     std::string err;
 };
 

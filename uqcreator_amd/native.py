@@ -71,6 +71,8 @@ SIGNATURES = {
                                       C.c_void_p]),
     "uqk_closure_solve_host": (C.c_int64, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.c_void_p]),
+    "uqk_closure_solve_host_noalloc": (C.c_int64, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                   C.c_void_p, C.c_void_p]),
     "uqk_eval_uq": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
     "uqk_reconstruct": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
     "uqk_compute_dt": (C.c_int, [C.c_void_p, C.POINTER(ProblemDesc), C.POINTER(MeshDesc), C.c_void_p,
@@ -353,6 +355,18 @@ class GpuClosure:
         ntc = None if nt_cell is None else np.ascontiguousarray(nt_cell, dtype=np.int32)
         self._check(lib().uqk_closure_solve_host(self.h, n, _p(ntc), _p(u), _p(lam), _p(status), _p(iters)),
                     "uqk_closure_solve_host")
+        return status, iters
+
+    def solve_host_noalloc(self, u, lam, nt_cell=None):
+        """solve_host through the allocation-free entry point (device staging
+        in the library's fixed per-GPU array). Returns (status, iters)."""
+        n = u.shape[0]
+        assert u.flags.c_contiguous and lam.flags.c_contiguous
+        status = np.zeros(n, dtype=np.int32)
+        iters = np.zeros(n, dtype=np.int32)
+        ntc = None if nt_cell is None else np.ascontiguousarray(nt_cell, dtype=np.int32)
+        self._check(lib().uqk_closure_solve_host_noalloc(self.h, n, _p(ntc), _p(u), _p(lam), _p(status), _p(iters)),
+                    "uqk_closure_solve_host_noalloc")
         return status, iters
 
     def eval_uq(self, lam, uq, nt_cell=None):
