@@ -241,8 +241,17 @@ int32_t uqk_device_count(void);
  * uqk_step call, timed with HIP events on the launch stream. */
 double uqk_last_closure_ms(const uqk_ctx* ctx);
 /* Totals of the last closure call: Newton steps (Hessian factorisations) and
- * gradient evaluations (line-search trials included) over all cells. */
+ * gradient evaluations (line-search trials included) over all cells; after
+ * uqk_run_steps, the totals over the run's steps. */
 int uqk_last_closure_stats(const uqk_ctx* ctx, int64_t* newton_steps, int64_t* gradient_evaluations);
+/* Per-step kernel timing (no counterpart in the reference): while on, every
+ * step that uqk_step, or uqk_run_steps without a graph, enqueues brackets its
+ * closure kernel and its advance kernel with HIP events on the context's
+ * stream, and uqk_kernel_times returns the sums of their durations and the
+ * number of steps timed.  Turning it on or off resets the sums.  Steps
+ * replayed from a captured graph are not timed. */
+int uqk_set_kernel_timing(uqk_ctx* ctx, int32_t on);
+int uqk_kernel_times(const uqk_ctx* ctx, double* closure_ms, double* advance_ms, int64_t* steps);
 
 /* ---- launch shapes and the autotuner (no counterpart in the reference) --- */
 /* Launch shape of the specialised closure kernel: wavefronts (cells in

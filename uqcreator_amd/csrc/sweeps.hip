@@ -358,30 +358,36 @@ __global__ void __launch_bounds__(kSweepThreads) mean_var_kernel(SweepArgs a) {
 }
 
 // t < t_end and steps < max_steps and no failed cell: the step runs
-__global__ void step_begin_kernel(RunState* rs, double t_end, long long max_steps) {
-    if (threadIdx.x == 0 && blockIdx.x == 0)
-        rs->active = (rs->t < t_end && (max_steps < 0 || rs->steps < max_steps) && rs->nfail == 0) ? 1 : 0;
-}
-
-// t += dt (momentsolver.cpp:134), counters of the step into the run totals
-__global__ void step_end_kernel(RunState* rs, const double* scalars, const unsigned long long* counters) {
-    if (threadIdx.x == 0 && blockIdx.x == 0 && rs->active) {
-        rs->t += scalars[0];
-        rs->residual = scalars[1];
-        rs->steps += 1;
-        rs->nfail += (long long)counters[1];
+// The end of a replayed step: residual and t (momentsolver.cpp:134-152), run
+// totals, and what the next step's launches need (zero closure counters, the
+// start value 1e10 of ComputeDt's MIN, the active flag).
+__global__ void __launch_bounds__(kSweepThreads) step_end_kernel(RunState* rs, double* scalars,
+                                                                 unsigned long long* counters,
+                                                                 const double* partial, int npartial) {
+    if (!rs->active) return;
+    double m = 0.0;
+    for (int t = threadIdx.x; t < npartial; t += blockDim.x) m += partial[t];
+    m = block_sum(m);
+    if (threadIdx.x == 0) {
+        const double t = rs->t + scalars[0];
+        const long long steps = rs->steps + 1;
+        const long long nfail = rs->nfail + (long long)counters[1];
+        rs->t = t;
+        rs->residual = m;
+        rs->steps = steps;
+        rs->nfail = nfail;
         rs->newton += (long long)counters[2];
         rs->gradients += (long long)counters[3];
+        counters[0] = counters[1] = counters[2] = counters[3] = 0ull;
+        scalars[0] = 1e10;
+        rs->active = (t < rs->t_end && (rs->max_steps < 0 || steps < rs->max_steps) && nfail == 0) ? 1 : 0;
     }
 }
 
-hipError_t launch_step_begin(RunState* rs, double t_end, long long max_steps, hipStream_t st) {
-    hipLaunchKernelGGL(step_begin_kernel, dim3(1), dim3(64), 0, st, rs, t_end, max_steps);
-    return hipGetLastError();
-}
-
-hipError_t launch_step_end(RunState* rs, const double* scalars, const unsigned long long* counters, hipStream_t st) {
-    hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(64), 0, st, rs, scalars, counters);
+hipError_t launch_step_end(RunState* rs, double* scalars, unsigned long long* counters, const double* partial,
+                           int npartial, hipStream_t st) {
+    hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(kSweepThreads), 0, st, rs, scalars, counters, partial,
+                       npartial);
     return hipGetLastError();
 }
 

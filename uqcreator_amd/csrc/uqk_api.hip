@@ -113,12 +113,14 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
                    int32_t* iters, const int* active = nullptr, double* uq_out = nullptr, double* u_out = nullptr,
                    bool* fused = nullptr, double* dt_out = nullptr, const uqk::ProblemParams* dtp = nullptr,
                    const double* dx_param = nullptr, bool* fused_dt = nullptr,
-                   unsigned long long* counters = nullptr) {
+                   unsigned long long* counters = nullptr, bool prezeroed = false) {
     if (fused) *fused = false;
     if (fused_dt) *fused_dt = false;
     ctx->timed = false;
     if (!counters) counters = ctx->d_counters;
-    UQK_HIP(ctx, uqk::launch_fill_u64(counters, 4, ctx->stream));  // d_counters[4..7] belong to the warm start
+    // (in a replayed run the previous step's step_end_kernel already cleared
+    // the counters and set dt's start value: prezeroed)
+    if (!prezeroed) UQK_HIP(ctx, uqk::launch_fill_u64(counters, 4, ctx->stream));  // d_counters[4..7]: warm start
     if (n <= 0) return 0;
     if (ctx->warm && !nt) {
         int rc = uqk::enqueue_warm_start(ctx, n, u, lam, active);
@@ -144,22 +146,25 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     if (want_dt) {
         // the start value of ComputeDt's MIN (momentsolver.cpp:127); a path
         // that does not fuse leaves dt to the dt sweep, which overwrites it
-        UQK_HIP(ctx, uqk::launch_fill_f64(dt_out, 1, 1e10, ctx->stream));
+        if (!prezeroed) UQK_HIP(ctx, uqk::launch_fill_f64(dt_out, 1, 1e10, ctx->stream));
         a.dtp = *dtp;
         a.dx_param = dx_param;
         a.dt_min = reinterpret_cast<unsigned long long*>(dt_out);
     }
     const bool ev = !ctx->capturing;
+    // the step timer brings its own pair (uqk_set_kernel_timing), else ev0/ev1
+    ctx->tev0 = ctx->cev0 ? ctx->cev0 : ctx->ev0;
+    ctx->tev1 = ctx->cev1 ? ctx->cev1 : ctx->ev1;
     if (ctx->kernel == UQK_KERNEL_AUTO || ctx->kernel == UQK_KERNEL_WAVE) {
         hipError_t e = hipSuccess;
         const bool fuse = uq_out && u_out && ctx->c.max_iter != 1;
         a.uq_out = fuse ? uq_out : nullptr;
         a.u_out = fuse ? u_out : nullptr;
-        if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev0, ctx->stream));
         if (uqk::newton_wave_launch(ctx->kind, ctx->M, ctx->Q, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream, &e,
                                     ctx->shape)) {
             UQK_HIP(ctx, e);
-            if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+            if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev1, ctx->stream));
             ctx->last_kernel = UQK_KERNEL_WAVE;
             ctx->timed = ev;
             if (fused) *fused = fuse;
@@ -174,9 +179,9 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         const bool fuse = uq_out && u_out && ctx->c.max_iter != 1;
         a.uq_out = fuse ? uq_out : nullptr;
         a.u_out = fuse ? u_out : nullptr;
-        if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev0, ctx->stream));
         UQK_HIP(ctx, uqk::newton_block_launch(ctx->kind, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream));
-        if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev1, ctx->stream));
         ctx->last_kernel = UQK_KERNEL_BLOCK;
         ctx->timed = ev;
         if (fused) *fused = fuse;
@@ -215,27 +220,46 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     if (occ <= 0) occ = 1;
     long want = (n + waves - 1) / waves;
     int blocks = (int)std::min<long>(want, (long)occ * ctx->num_cu);
-    if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev0, ctx->stream));
     UQK_HIP(ctx, uqk::newton_fp64_launch(ctx->kind, a, tlds, blocks, threads, shmem, ctx->stream));
-    if (ev) UQK_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev1, ctx->stream));
     ctx->timed = ev;
     return 0;
 }
 
 // Enqueue one step body (momentsolver.cpp:83-152) on ctx->stream without any
 // host synchronisation: closure, uQ, reconstruction, dt, advance, residual.
+// in_run: a step of uqk_run_steps, whose step_end_kernel clears the counters,
+// sets dt's start value and sums the residual (no fills, no residual kernel).
 // With refinement on (ctx->refine.nlev > 1) the levels move between uQ and the
 // reconstruction, as in momentsolver.cpp:99-124: the closure and uQ use the
 // old nTotal, the reconstruction and the advance the new one.
 int enqueue_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, int32_t* nt, double* u,
-                 double* lam, double* uq, double* u_scratch, int32_t* status, int32_t* iters, const int* active) {
+                 double* lam, double* uq, double* u_scratch, int32_t* status, int32_t* iters, const int* active,
+                 bool in_run = false) {
     const int64_t n = mesh->n_cells;
     const bool refining = ctx->refine.nlev > 1;
     if (refining && !nt) return fail(ctx, -1, "refinement levels need ntotal_cell");
     bool fused = false, fused_dt = false;
     const uqk::ProblemParams pp = problem_params(prob);
+    // per-step kernel timing: four pool events (closure and advance pairs)
+    const bool kt = ctx->ktiming && !ctx->capturing && n > 0;
+    hipEvent_t ea0 = nullptr, ea1 = nullptr;
+    if (kt) {
+        while (ctx->kev.size() < ctx->kev_used + 4) {
+            hipEvent_t e = nullptr;
+            UQK_HIP(ctx, hipEventCreate(&e));
+            ctx->kev.push_back(e);
+        }
+        ctx->cev0 = ctx->kev[ctx->kev_used];
+        ctx->cev1 = ctx->kev[ctx->kev_used + 1];
+        ea0 = ctx->kev[ctx->kev_used + 2];
+        ea1 = ctx->kev[ctx->kev_used + 3];
+    }
     int rc = launch_closure(ctx, n, nt, u, lam, status, iters, active, refining ? nullptr : uq,
-                            refining ? nullptr : u_scratch, &fused, ctx->d_scalars, &pp, mesh->dx_param, &fused_dt);
+                            refining ? nullptr : u_scratch, &fused, ctx->d_scalars, &pp, mesh->dx_param, &fused_dt,
+                            nullptr, in_run);
+    ctx->cev0 = ctx->cev1 = nullptr;
     if (rc) return rc;
     uqk::SweepArgs a = sweep_args(ctx);
     a.p = pp;
@@ -262,9 +286,31 @@ int enqueue_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, in
     a.u = u_scratch;
     a.u_out = u;
     a.dt_dev = ctx->d_scalars;
+    if (kt) UQK_HIP(ctx, hipEventRecord(ea0, ctx->stream));
     UQK_HIP(ctx, uqk::launch_advance(ctx->kind, a, ctx->stream));
-    a.scalar_out = ctx->d_scalars + 1;
-    UQK_HIP(ctx, uqk::launch_residual(a, ctx->stream));
+    if (kt) {
+        UQK_HIP(ctx, hipEventRecord(ea1, ctx->stream));
+        ctx->kev_used += 4;
+    }
+    if (!in_run) {
+        a.scalar_out = ctx->d_scalars + 1;
+        UQK_HIP(ctx, uqk::launch_residual(a, ctx->stream));
+    }
+    return 0;
+}
+
+// Adds the kernel times of the steps enqueued since the last call (their
+// events have completed: the stream was synchronised) to the step timer.
+int collect_step_times(uqk_ctx* ctx) {
+    for (size_t i = 0; i + 4 <= ctx->kev_used; i += 4) {
+        float c = 0.f, a = 0.f;
+        UQK_HIP(ctx, hipEventElapsedTime(&c, ctx->kev[i], ctx->kev[i + 1]));
+        UQK_HIP(ctx, hipEventElapsedTime(&a, ctx->kev[i + 2], ctx->kev[i + 3]));
+        ctx->k_closure_ms += c;
+        ctx->k_advance_ms += a;
+        ++ctx->k_steps;
+    }
+    ctx->kev_used = 0;
     return 0;
 }
 
@@ -275,7 +321,7 @@ int64_t finish_closure(uqk_ctx* ctx, const unsigned long long* counters = nullpt
     UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
     ctx->last_closure_ms = 0.0;
-    if (ctx->timed) UQK_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (ctx->timed) UQK_HIP(ctx, hipEventElapsedTime(&ms, ctx->tev0, ctx->tev1));
     if (ctx->timed) ctx->last_closure_ms = ms;
     return (int64_t)ctx->last_stats[0];
 }
@@ -397,6 +443,7 @@ int uqk_destroy(uqk_ctx* ctx) {
     if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
     hipFree(ctx->d_run);
     if (ctx->h_run) hipHostFree(ctx->h_run);
+    for (hipEvent_t e : ctx->kev) hipEventDestroy(e);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
@@ -659,6 +706,8 @@ int64_t uqk_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, in
     double sc[2];
     UQK_HIP(ctx, hipMemcpyAsync(sc, ctx->d_scalars, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     int64_t nfail = finish_closure(ctx);
+    if (nfail >= 0)
+        if (int rc2 = collect_step_times(ctx)) return rc2;
     if (dt_out) *dt_out = sc[0];
     if (residual_out) *residual_out = sc[1];
     return nfail;
@@ -678,14 +727,20 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
     UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *ctx->h_run = uqk::RunState{};
     ctx->h_run->t = t0;
-    ctx->h_run->active = 1;
+    ctx->h_run->t_end = t_end;
+    ctx->h_run->max_steps = max_steps;
+    ctx->h_run->active = (t0 < t_end && max_steps != 0) ? 1 : 0;
     UQK_HIP(ctx, hipMemcpy(ctx->d_run, ctx->h_run, sizeof(uqk::RunState), hipMemcpyHostToDevice));
+    // what each step's step_end_kernel leaves for the next one: zero closure
+    // counters and dt's start value (momentsolver.cpp:127)
+    UQK_HIP(ctx, uqk::launch_fill_u64(ctx->d_counters, 4, ctx->stream));
+    UQK_HIP(ctx, uqk::launch_fill_f64(ctx->d_scalars, 1, 1e10, ctx->stream));
     const int* active = &ctx->d_run->active;
     auto enqueue = [&]() -> int {
-        UQK_HIP(ctx, uqk::launch_step_begin(ctx->d_run, t_end, max_steps, ctx->stream));
-        int rc = enqueue_step(ctx, prob, mesh, nt, u, lam, uq, u_scratch, status, iters, active);
+        int rc = enqueue_step(ctx, prob, mesh, nt, u, lam, uq, u_scratch, status, iters, active, true);
         if (rc) return rc;
-        UQK_HIP(ctx, uqk::launch_step_end(ctx->d_run, ctx->d_scalars, ctx->d_counters, ctx->stream));
+        UQK_HIP(ctx, uqk::launch_step_end(ctx->d_run, ctx->d_scalars, ctx->d_counters, ctx->d_partial,
+                                          uqk::sweep_partial_blocks(0), ctx->stream));
         return 0;
     };
     if (use_graph) {
@@ -693,14 +748,14 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
         // (the setters of that state also drop the graph)
         char key[640];
         snprintf(key, sizeof key,
-                 "%d %d %.17g %.17g %.17g %.17g %d|%lld %d %d %p %p %p %p %p %p %p|%p %p %p %p %p %p %p|%.17g %lld|"
+                 "%d %d %.17g %.17g %.17g %.17g %d|%lld %d %d %p %p %p %p %p %p %p|%p %p %p %p %p %p %p|"
                  "%d %d %d %p %p",
                  prob->problem, prob->flux, prob->cfl, prob->gamma, prob->gravity, prob->friction, prob->friction_dim,
                  (long long)mesh->n_cells, mesh->n_edges, mesh->dim, (const void*)mesh->neighbor,
                  (const void*)mesh->unit_normal, (const void*)mesh->normal, (const void*)mesh->area,
                  (const void*)mesh->dx_param, (const void*)mesh->bc, (const void*)mesh->boundary_normal,
                  (const void*)nt, (void*)u, (void*)lam, (void*)uq, (void*)u_scratch, (void*)status, (void*)iters,
-                 t_end, (long long)max_steps, ctx->kernel, ctx->shape.waves, ctx->shape.blocks_per_cu,
+                 ctx->kernel, ctx->shape.waves, ctx->shape.blocks_per_cu,
                  (void*)ctx->warm, (void*)ctx->stream);
         if (!ctx->graph_exec || ctx->graph_key != key) {
             if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
@@ -741,9 +796,13 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
         launched += todo;
         UQK_HIP(ctx, hipMemcpyAsync(ctx->h_run, ctx->d_run, sizeof(uqk::RunState), hipMemcpyDeviceToHost, ctx->stream));
         UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (int rc = collect_step_times(ctx)) return rc;
         if (!rs.active || launched >= limit || rs.nfail) break;
     }
     ctx->timed = false;
+    ctx->last_stats[0] = (unsigned long long)rs.nfail;  // uqk_last_closure_stats: the run's totals
+    ctx->last_stats[1] = (unsigned long long)rs.newton;
+    ctx->last_stats[2] = (unsigned long long)rs.gradients;
     if (t_out) *t_out = rs.t;
     if (steps_out) *steps_out = rs.steps;
     if (residual_out) *residual_out = rs.residual;
@@ -769,6 +828,24 @@ int uqk_last_closure_stats(const uqk_ctx* ctx, int64_t* newton_steps, int64_t* g
     if (!ctx) return -1;
     if (newton_steps) *newton_steps = (int64_t)ctx->last_stats[1];
     if (gradient_evaluations) *gradient_evaluations = (int64_t)ctx->last_stats[2];
+    return 0;
+}
+
+int uqk_set_kernel_timing(uqk_ctx* ctx, int32_t on) {
+    if (!ctx) return -1;
+    if (ctx->kev_used) UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->kev_used = 0;
+    ctx->ktiming = on != 0;
+    ctx->k_closure_ms = ctx->k_advance_ms = 0.0;
+    ctx->k_steps = 0;
+    return 0;
+}
+
+int uqk_kernel_times(const uqk_ctx* ctx, double* closure_ms, double* advance_ms, int64_t* steps) {
+    if (!ctx) return -1;
+    if (closure_ms) *closure_ms = ctx->k_closure_ms;
+    if (advance_ms) *advance_ms = ctx->k_advance_ms;
+    if (steps) *steps = ctx->k_steps;
     return 0;
 }
 

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "uqk.h"
 #include "uqk_internal.h"
@@ -33,7 +34,16 @@ struct uqk_ctx {
     unsigned long long last_stats[3] = {0, 0, 0};
     int kernel = UQK_KERNEL_AUTO;
     int last_kernel = UQK_KERNEL_AUTO;
-    bool timed = false;  // ev0/ev1 bracket the last closure kernel
+    bool timed = false;  // tev0/tev1 bracket the last closure kernel
+    hipEvent_t tev0 = nullptr, tev1 = nullptr;  // the pair the last closure recorded (ev0/ev1 or the step timer's)
+    // per-step kernel timing (uqk_set_kernel_timing): four pool events per
+    // step enqueued since the last collection (closure pair, advance pair)
+    bool ktiming = false;
+    std::vector<hipEvent_t> kev;
+    size_t kev_used = 0;
+    hipEvent_t cev0 = nullptr, cev1 = nullptr;  // closure pair of the step being enqueued
+    double k_closure_ms = 0.0, k_advance_ms = 0.0;
+    long long k_steps = 0;
     bool capturing = false;  // enqueueing into a graph: no events, no syncs
     uqk::RunState* d_run = nullptr;
     uqk::RunState* h_run = nullptr;  // pinned host copy of *d_run, read back by the replay loop

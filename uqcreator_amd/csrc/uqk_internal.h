@@ -39,7 +39,10 @@ struct NewtonArgs {
     unsigned long long* dt_min;
 };
 
-// Device-side state of a multi-step run replayed from a captured graph.
+// Device-side state of a multi-step run replayed from a captured graph.  The
+// host writes t, t_end, max_steps and the first step's active flag; each
+// step's step_end_kernel sets the flag for the next step, so the captured
+// launches do not depend on the run's length.
 struct RunState {
     double t;
     double residual;
@@ -47,6 +50,8 @@ struct RunState {
     long long newton;
     long long gradients;
     long long nfail;
+    double t_end;
+    long long max_steps;  // < 0: no limit
     int active;
 };
 
@@ -139,7 +144,11 @@ hipError_t launch_mean_var(int kind, const SweepArgs& a, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, int n, hipStream_t st);
 hipError_t launch_fill_f64(double* p, int n, double v, hipStream_t st);
 int sweep_partial_blocks(long ncell);
-hipError_t launch_step_begin(RunState* rs, double t_end, long long max_steps, hipStream_t st);
-hipError_t launch_step_end(RunState* rs, const double* scalars, const unsigned long long* counters, hipStream_t st);
+// End of a replayed step (momentsolver.cpp:134-152): residual = sum of the
+// advance's per-block partials, t += dt, the step's closure counts into the
+// run totals; then clears the closure counters, sets the fused dt's start value
+// and the next step's active flag.
+hipError_t launch_step_end(RunState* rs, double* scalars, unsigned long long* counters, const double* partial,
+                           int npartial, hipStream_t st);
 
 }  // namespace uqk

@@ -100,6 +100,8 @@ SIGNATURES = {
                                      C.c_void_p, C.c_void_p]),
     "uqk_last_closure_ms": (C.c_double, [C.c_void_p]),
     "uqk_last_closure_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "uqk_set_kernel_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+    "uqk_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "uqk_synth_euler1d_closure": (C.c_int64, [C.c_int64, C.c_uint64, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                                               C.c_void_p, C.c_void_p, C.c_void_p]),
     "uqk_synth_euler1d_closure_host": (C.c_int64, [C.c_int64, C.c_uint64, C.c_int32, C.c_int32, C.c_void_p,
@@ -193,9 +195,17 @@ class DeviceArray:
             raise RuntimeError("device->host copy failed")
         return out
 
-    def copy_from(self, other):
-        assert other.nbytes == self.nbytes
-        if lib().uqk_memcpy(C.c_void_p(self.ptr), C.c_void_p(other.ptr), self.nbytes, 2) != 0:
+    def copy_from(self, other, dst_offset=None):
+        """Device copy of `other`: the whole array, or (dst_offset given) into
+        the rows dst_offset.. of the first dimension."""
+        if dst_offset is None:
+            assert other.nbytes == self.nbytes
+            dst = self.ptr
+        else:
+            row = self.nbytes // self.shape[0]
+            assert other.nbytes % row == 0 and 0 <= dst_offset and dst_offset * row + other.nbytes <= self.nbytes
+            dst = self.ptr + dst_offset * row
+        if lib().uqk_memcpy(C.c_void_p(dst), C.c_void_p(other.ptr), other.nbytes, 2) != 0:
             raise RuntimeError("device->device copy failed")
 
     def free(self):
@@ -427,6 +437,17 @@ class GpuClosure:
         a, b = C.c_int64(), C.c_int64()
         lib().uqk_last_closure_stats(self.h, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def set_kernel_timing(self, on=True):
+        """Per-step HIP-event timing of the closure and advance kernels of
+        steps enqueued without a graph; resets the sums."""
+        self._check(lib().uqk_set_kernel_timing(self.h, int(bool(on))), "uqk_set_kernel_timing")
+
+    def kernel_times(self):
+        """(closure ms, advance ms, steps) summed since timing was turned on."""
+        c, a, n = C.c_double(), C.c_double(), C.c_int64()
+        lib().uqk_kernel_times(self.h, C.byref(c), C.byref(a), C.byref(n))
+        return c.value, a.value, n.value
 
 
 def problem(kind, flux=FLUX_REFERENCE, cfl=0.5, gamma=1.4, gravity=9.81, friction=0.0, friction_dim=-1):
