@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
     // is cell bitreverse(t) (skipped when >= ncell): neighbouring costly cells
     // go to different chunks.  Newton and failure counts are summed per wave
     // (wave-uniform, scalar) and added once at the end.
-    unsigned long long w_fail = 0, w_newton = 0, w_grad = 0;
+    unsigned long long w_fail = 0, w_newton = 0, w_grad = 0, w_took = 0;
     double w_dt = 1e10;  // min ComputeDt over this wave's cells (fused path)
     const long qsize = 1L << a.qbits;
     long qpos = 0, qend = 0;
@@ -478,18 +478,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                         else if (cc == N && r < N) yb[r] = T[ra][i][tb];
                     }
             wave_sync();
-            // U x = y with lane r owning x_r: row r of U and 1/U_rr are read
-            // into registers first, so the chain of a step is one multiply,
-            // one SGPR broadcast and one FMA.
+            // U x = y with lane r owning x_r: the chain of a step is one
+            // multiply, one SGPR broadcast and one FMA; column i of U comes
+            // from LDS, read ahead of the chain (its loads do not depend on it)
             double yr = lane < N ? yb[lane] : 0.0;
             const double rd = lane < N ? rdiag[lane] : 0.0;
-            double ucol[N];
-#pragma unroll
-            for (int i = 0; i < N; ++i) ucol[i] = lane < i ? Hs[i * (i + 1) / 2 + lane] : 0.0;
 #pragma unroll
             for (int i = N - 1; i >= 0; --i) {
+                const double ui = lane < i ? Hs[i * (i + 1) / 2 + lane] : 0.0;
                 const double xi = readlane_f64(yr * rd, i);
-                yr = lane == i ? xi : fma(-ucol[i], xi, yr);
+                yr = lane == i ? xi : fma(-ui, xi, yr);
             }
             d = yr;
             wave_sync();
@@ -504,6 +502,32 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
         double ng = gradient(lam, g);
         int status = -1;
         int iters = 0;
+        if (a.lam_alt && (c.safe || !(ng < c.eps))) {
+            // learned start: Newton starts from the candidate with the smaller
+            // ||grad L|| (a NaN norm fails the comparison and never wins)
+            const double* la = a.lam_alt + cell * SM;
+            for (int t = lane; t < SM; t += 64) lamnew[t] = la[t];
+            double Lk0[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) Lk0[s] = Lk[s];
+            const double g0 = g;
+            double galt = 0.0;
+            wave_sync();
+            ++ngrad;
+            const double nalt = gradient(lamnew, galt);
+            if (nalt < ng) {
+                double* t = lam;
+                lam = lamnew;
+                lamnew = t;
+                g = galt;
+                ng = nalt;
+                ++w_took;
+            } else {
+                g = g0;
+#pragma unroll
+                for (int s = 0; s < S; ++s) Lk[s] = Lk0[s];
+            }
+        }
         if (!c.safe && ng < c.eps) status = UQK_STATUS_CONVERGED;
         for (int it = 0; status < 0 && it < c.max_iter; ++it) {
             if (it > 0) g = gnew;
@@ -598,6 +622,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
         if (w_fail) atomicAdd(a.nfail, w_fail);
         if (w_newton) atomicAdd(a.nfail + 1, w_newton);
         if (w_grad) atomicAdd(a.nfail + 2, w_grad);
+        if (w_took && a.took) atomicAdd(a.took, w_took);
         // positive doubles order as their bit patterns
         if (a.dt_min && w_dt < 1e10) atomicMin(a.dt_min, (unsigned long long)__double_as_longlong(w_dt));
     }

@@ -305,74 +305,129 @@ struct FeatArgs {
     float* tf;                // rows x NO, targets (raw if mu_t null)
 };
 
+// Per-row scales and linearised start of a block's rows, staged in LDS by
+// the first kRowsPerBlock threads (one row each) for the block's other
+// threads, which then spread over (row, feature) pairs.
+constexpr int kRowsPerBlock = 64;
+template <int S>
+struct RowScales {
+    double D[kRowsPerBlock][S];
+    double ds[kRowsPerBlock][S];
+    double L[kRowsPerBlock][S][S];
+    int ok[kRowsPerBlock];
+};
+
 template <int KIND>
-__global__ void features_kernel(FeatArgs a) {
+__device__ void stage_rows(RowScales<uqk::KindInfo<KIND>::S>& rs, long n, long r0, const long* idx,
+                           const double* u, int M, const uqk::ClosureParams& cp) {
     constexpr int S = uqk::KindInfo<KIND>::S;
-    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= a.n) return;
-    const long c = a.idx ? a.idx[r] : r;
-    const double* u = a.u + (size_t)c * a.M * S;
-    double D[S], ds[S];
+    const int t = threadIdx.x;
+    if (t < kRowsPerBlock) {
+        const long r = r0 + t;
+        int ok = 0;
+        if (r < n) {
+            const long c = idx ? idx[r] : r;
+            double D[S], ds[S];
+            LinearStart<KIND> lin;
+            ok = euler_scales<S>(u + (size_t)c * M * S, cp.gamma, D, ds) && lin.factor(cp, ds);
+            for (int s = 0; s < S; ++s) {
+                rs.D[t][s] = D[s];
+                rs.ds[t][s] = ds[s];
+                for (int q = 0; q < S; ++q) rs.L[t][s][q] = lin.L[s][q];
+            }
+        }
+        rs.ok[t] = ok;
+    }
+    __syncthreads();
+}
+
+// z = the linearised start of moment i of block row `row` (i = 0: DS(u_0)).
+template <int KIND>
+__device__ void linear_start_z(const RowScales<uqk::KindInfo<KIND>::S>& rs, int row, int i, const double* ui,
+                               double* z) {
+    constexpr int S = uqk::KindInfo<KIND>::S;
+    if (i == 0) {
+        for (int s = 0; s < S; ++s) z[s] = rs.ds[row][s];
+        return;
+    }
     LinearStart<KIND> lin;
-    const bool ok = euler_scales<S>(u, a.cp.gamma, D, ds) && lin.factor(a.cp, ds);
+    for (int s = 0; s < S; ++s)
+        for (int q = 0; q < S; ++q) lin.L[s][q] = rs.L[row][s][q];
+    lin.solve(ui, z);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void features_kernel(FeatArgs a) {
+    constexpr int S = uqk::KindInfo<KIND>::S;
+    __shared__ RowScales<S> rs;
+    const long r0 = (long)blockIdx.x * kRowsPerBlock;
+    stage_rows<KIND>(rs, a.n, r0, a.idx, a.u, a.M, a.cp);
     const int nf = a.M * S;
-    for (int f = 0; f < a.K0; ++f) {
+    // features x_f = u_f / D_s, standardised; consecutive threads write
+    // consecutive features of a row
+    for (int t = threadIdx.x; t < kRowsPerBlock * a.K0; t += blockDim.x) {
+        const int row = t / a.K0, f = t - row * a.K0;
+        const long r = r0 + row;
+        if (r >= a.n) break;
         float x = 0.f;
-        if (ok && f < nf) {
-            x = (float)(u[f] / D[f % S]);
+        if (rs.ok[row] && f < nf) {
+            const long c = a.idx ? a.idx[r] : r;
+            x = (float)(a.u[(size_t)c * nf + f] / rs.D[row][f % S]);
             if (a.mu_x) x = (x - a.mu_x[f]) / a.sd_x[f];
         }
         if (a.xh) a.xh[(size_t)r * a.K0 + f] = (__bf16)x;
         if (a.xf) a.xf[(size_t)r * a.K0 + f] = x;
     }
     if (!a.lam) return;
-    const double* l = a.lam + (size_t)c * a.M * S;
-    float* tr = a.tf + (size_t)r * a.NO;
-    for (int i = 0; i < a.M; ++i) {
+    // targets: the correction to the linearised start, scaled by D_s / rho_0
+    for (int t = threadIdx.x; t < kRowsPerBlock * a.M; t += blockDim.x) {
+        const int row = t / a.M, i = t - row * a.M;
+        const long r = r0 + row;
+        if (r >= a.n) break;
+        const long c = a.idx ? a.idx[r] : r;
+        const double* u = a.u + (size_t)c * nf;
+        const double* l = a.lam + (size_t)c * nf;
+        float* tr = a.tf + (size_t)r * a.NO;
         double z[S];
-        if (i == 0) {
-            for (int s = 0; s < S; ++s) z[s] = ds[s];
-        } else if (ok) {
-            lin.solve(u + i * S, z);
-        }
+        if (rs.ok[row]) linear_start_z<KIND>(rs, row, i, u + i * S, z);
         for (int s = 0; s < S; ++s) {
             const int f = i * S + s;
-            float t = 0.f;
-            if (ok) {
-                t = (float)((l[f] - z[s]) * D[s] / D[0]);
-                if (a.mu_t) t = (t - a.mu_t[f]) / a.sd_t[f];
+            float tv = 0.f;
+            if (rs.ok[row]) {
+                tv = (float)((l[f] - z[s]) * rs.D[row][s] / rs.D[row][0]);
+                if (a.mu_t) tv = (tv - a.mu_t[f]) / a.sd_t[f];
             }
-            tr[f] = t;
+            tr[f] = tv;
         }
+        if (i == 0)
+            for (int f = nf; f < a.NO; ++f) tr[f] = 0.f;
     }
-    for (int f = nf; f < a.NO; ++f) tr[f] = 0.f;
 }
 
 // lambda = lambda_lin + (mu_t + sd_t y) rho0/D_s; NaN where the mean state is
-// not physical (the selection then keeps the plain start).
+// not physical (the selection then keeps the plain start).  A block covers
+// kRowsPerBlock cells; threads over (cell, moment) pairs.
 template <int KIND>
-__global__ void apply_kernel(long n, const double* u, const float* y, int M, uqk::ClosureParams cp, int NO,
-                             const float* mu_t, const float* sd_t, double* lam) {
+__global__ __launch_bounds__(256) void apply_kernel(long n, const double* u, const float* y, int M,
+                                                    uqk::ClosureParams cp, int NO, const float* mu_t,
+                                                    const float* sd_t, double* lam) {
     constexpr int S = uqk::KindInfo<KIND>::S;
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    const double* uc = u + (size_t)c * M * S;
-    double D[S], ds[S];
-    LinearStart<KIND> lin;
-    const bool ok = euler_scales<S>(uc, cp.gamma, D, ds) && lin.factor(cp, ds);
-    for (int i = 0; i < M; ++i) {
+    __shared__ RowScales<S> rs;
+    const long c0 = (long)blockIdx.x * kRowsPerBlock;
+    stage_rows<KIND>(rs, n, c0, nullptr, u, M, cp);
+    for (int t = threadIdx.x; t < kRowsPerBlock * M; t += blockDim.x) {
+        const int row = t / M, i = t - row * M;
+        const long c = c0 + row;
+        if (c >= n) break;
+        const double* uc = u + (size_t)c * M * S;
         double z[S];
-        if (i == 0) {
-            for (int s = 0; s < S; ++s) z[s] = ds[s];
-        } else if (ok) {
-            lin.solve(uc + i * S, z);
-        }
+        if (rs.ok[row]) linear_start_z<KIND>(rs, row, i, uc + i * S, z);
         for (int s = 0; s < S; ++s) {
             const int f = i * S + s;
             double v = NAN;
-            if (ok) {
-                const double t = (double)mu_t[f] + (double)sd_t[f] * (double)y[(size_t)c * NO + f];
-                v = z[s] + t * D[0] / D[s];
+            if (rs.ok[row]) {
+                const double tv = (double)mu_t[f] + (double)sd_t[f] * (double)y[(size_t)c * NO + f];
+                v = z[s] + tv * rs.D[row][0] / rs.D[row][s];
             }
             lam[(size_t)c * M * S + f] = v;
         }
@@ -668,9 +723,11 @@ int launch_features(uqk_mlp* n, long rows, const long* idx, const double* u, con
     a.xf = xf;
     a.tf = tf;
     if (n->ctx->kind == UQK_EULER1D)
-        hipLaunchKernelGGL(features_kernel<UQK_EULER1D>, dim3(blocks(rows)), dim3(256), 0, n->ctx->stream, a);
+        hipLaunchKernelGGL(features_kernel<UQK_EULER1D>, dim3(blocks(rows, kRowsPerBlock)), dim3(256), 0,
+                           n->ctx->stream, a);
     else
-        hipLaunchKernelGGL(features_kernel<UQK_EULER2D>, dim3(blocks(rows)), dim3(256), 0, n->ctx->stream, a);
+        hipLaunchKernelGGL(features_kernel<UQK_EULER2D>, dim3(blocks(rows, kRowsPerBlock)), dim3(256), 0,
+                           n->ctx->stream, a);
     MLP_HIP(n, hipGetLastError());
     return 0;
 }
@@ -694,10 +751,10 @@ int enqueue_predict(uqk_mlp* n, long ncell, const double* u, double* lambda_out)
         if (int rc = forward(n, rows, false)) return rc;
         double* lc = lambda_out + (size_t)c0 * n->M * n->S;
         if (n->ctx->kind == UQK_EULER1D)
-            hipLaunchKernelGGL(apply_kernel<UQK_EULER1D>, dim3(blocks(rows)), dim3(256), 0, st, rows, uc, n->y,
+            hipLaunchKernelGGL(apply_kernel<UQK_EULER1D>, dim3(blocks(rows, kRowsPerBlock)), dim3(256), 0, st, rows, uc, n->y,
                                n->M, n->ctx->c, n->NO, mu_t, sd_t, lc);
         else
-            hipLaunchKernelGGL(apply_kernel<UQK_EULER2D>, dim3(blocks(rows)), dim3(256), 0, st, rows, uc, n->y,
+            hipLaunchKernelGGL(apply_kernel<UQK_EULER2D>, dim3(blocks(rows, kRowsPerBlock)), dim3(256), 0, st, rows, uc, n->y,
                                n->M, n->ctx->c, n->NO, mu_t, sd_t, lc);
     }
     MLP_HIP(n, hipGetLastError());
@@ -742,12 +799,14 @@ int prepare_warm_start(uqk_ctx* ctx, long ncell) {
     return ensure_rows(n, std::min<long>(ncell, kPredictChunk));
 }
 
-int enqueue_warm_start(uqk_ctx* ctx, long ncell, const double* u, double* lambda, const int* active) {
+int enqueue_warm_predict(uqk_ctx* ctx, long ncell, const double* u, const double** lam_alt) {
     uqk_mlp* n = ctx->warm;
+    *lam_alt = nullptr;
     if (!n) return 0;
     if (int rc = prepare_warm_start(ctx, ncell)) return rc;
     if (int rc = enqueue_predict(n, ncell, u, n->lam_alt)) return rc;
-    return enqueue_warm_select(ctx, ncell, u, lambda, n->lam_alt, active);
+    *lam_alt = n->lam_alt;
+    return 0;
 }
 
 }  // namespace uqk

@@ -150,147 +150,169 @@ __global__ void __launch_bounds__(kSweepThreads) sum_finish_kernel(SweepArgs a) 
     if (threadIdx.x == 0) *a.scalar_out = m;
 }
 
-// LDS doubles per wavefront of the advance: E x Q point fluxes of S states,
-// at least Q summed fluxes and the M x S projected ones.
-__host__ __device__ inline int advance_wave_doubles(int S, int E, int Q, int M) {
-    return (E * Q > Q + M ? E * Q : Q + M) * S;
+// Cells per wavefront of the advance: lanes take (cell slot, quadrature
+// point) for Q <= 32, as many cells as 64 lanes and the 16 rows of one MFMA
+// tile (cells x states) allow; one cell with ceil(Q/64) points per lane else.
+__host__ __device__ inline int advance_cells_per_wave(int S, int Q) {
+    if (Q > 32) return 1;
+    const int by_lanes = 64 / Q, by_rows = 16 / S;
+    return by_lanes < by_rows ? by_lanes : by_rows;
 }
 
-// One wavefront per cell.  Lanes over quadrature points sum the point fluxes of
-// all edges (the projection onto phiw is linear, so the edges share one
-// projection); lanes over (i, s) then project and update.  The residual
-// |u_new(0,0) - u_old(0,0)| * area of each cell is accumulated per block.
+// LDS doubles per wavefront of the advance: the edge-summed point fluxes of
+// its cells, (cell slot, point, state).
+__host__ __device__ inline int advance_wave_doubles(int S, int Q) {
+    return advance_cells_per_wave(S, Q) * Q * S;
+}
+
+// ExplicitEuler::Advance (expliciteuler.cpp:8-75) with numFlux
+// (momentsolver.cpp:413-416) for a wavefront's cells: each lane owns a
+// quadrature point of one cell and sums the point fluxes of all the cell's
+// edges in registers, in edge order (the projection onto phiw is linear, so
+// the edges share one projection); the shallow-water bed friction enters as
+// -area * S(u) at the point before the projection; the projection
+// rhs(s, i) = sum_k G(s, k) phiW(k, i) of all the wave's cells is one FP64
+// MFMA tile row block (rows: cell slot x state), and the lanes holding the
+// tile's elements update the moments.  The residual |u_new(0,0) - u_old(0,0)|
+// * area is accumulated per block.
+// (four waves per SIMD: the kernel waits on memory and square roots, and the
+// register budget of four waves holds it without spilling)
 template <int KIND>
-__global__ void __launch_bounds__(kSweepThreads) advance_kernel(SweepArgs a) {
+__global__ void __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(4))) advance_kernel(SweepArgs a) {
     constexpr int S = KindInfo<KIND>::S;
     UQK_SKIP_IF_INACTIVE(a)
     extern __shared__ double smem[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int M = a.M, Q = a.Q, SM = S * M, E = a.nedge, D = a.dim;
-    // per wave: point fluxes per (edge, point), later the projected fluxes
-    const int WD = advance_wave_doubles(S, E, Q, M);
-    double* Gs = smem + wave * WD;
-    // the projection table PhiW in LDS when it fits (launcher's choice)
+    const int CPW = advance_cells_per_wave(S, Q);
+    double* Gs = smem + wave * advance_wave_doubles(S, Q);
     const bool wlds = a.table_lds != 0;
-    double* wt = smem + (blockDim.x >> 6) * WD;
+    double* wt = smem + (blockDim.x >> 6) * advance_wave_doubles(S, Q);
     if (wlds) {
         for (int t = threadIdx.x; t < Q * M; t += blockDim.x) wt[t] = a.phiw[t];
         __syncthreads();
     }
     const double* phiw = wlds ? wt : a.phiw;
     const double dt = a.dt_dev ? *a.dt_dev : a.dt;
+    const bool fric = (S == 3 && a.p.problem == UQK_SHALLOW_WATER2D_PROBLEM && a.p.friction > 0.0);
+    // this lane's cell slot and first point
+    const int slot = CPW > 1 ? lane / Q : 0;
+    const int k0 = CPW > 1 ? lane - slot * Q : lane;
+    const bool lane_on = slot < CPW && k0 < Q;
     double res = 0.0;
     const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
-    for (long cell = (long)blockIdx.x * (blockDim.x >> 6) + wave; cell < a.ncell; cell += nwaves) {
-        const int bc = a.bc[cell];
-        const double* us = a.u + cell * SM;  // reconstructed moments
-        double* un = a.u_out + cell * SM;    // in: previous moments, out: new moments
-        const double old00 = un[0];
-        if (bc == UQK_BC_DIRICHLET) {
-            for (int e = lane; e < SM; e += 64) un[e] = us[e];
-            if (lane == 0) res += fabs(us[0] - old00) * a.area[cell];
-            continue;
-        }
-        const int nt = a.nt_cell ? a.nt_cell[cell] : M;
-        const double* uqj = a.uq + cell * S * Q;
-        const bool wall = (bc == UQK_BC_NOSLIP || bc == UQK_BC_SWWALL);
-        // one point flux per lane and item (edge e, point k): every lane's
-        // loads are independent, so a cell's edges overlap their latencies
-        for (int t = lane; t < E * Q; t += 64) {
-            const int e = t / Q, k = t - e * Q;
-            const int nb = a.nbr[cell * E + e];
-            const double* nu = a.unit_normal + (cell * E + e) * D;
-            const double* nv = a.normal + (cell * E + e) * D;
-            const double nux = nu[0], nuy = D > 1 ? nu[1] : 0.0, nx = nv[0], ny = D > 1 ? nv[1] : 0.0;
-            double uj[S], G[S], v[S];
+    for (long cell0 = ((long)blockIdx.x * (blockDim.x >> 6) + wave) * CPW; cell0 < a.ncell; cell0 += nwaves * CPW) {
+        const long cell = cell0 + slot;
+        const bool on = lane_on && cell < a.ncell;
+        const int bc = on ? a.bc[cell] : UQK_BC_DIRICHLET;
+        if (on && bc != UQK_BC_DIRICHLET) {
+            const double* uqj = a.uq + cell * S * Q;
+            const bool wall = (bc == UQK_BC_NOSLIP || bc == UQK_BC_SWWALL);
+            const double area = a.area[cell];
+            for (int k = k0; k < Q; k += 64) {
+                double uj[S], gj[S], Gsum[S];
 #pragma unroll
-            for (int s = 0; s < S; ++s) uj[s] = uqj[k * S + s];
-            if (wall && nb == a.ncell) {
+                for (int s = 0; s < S; ++s) {
+                    uj[s] = uqj[k * S + s];
+                    Gsum[s] = 0.0;
+                }
                 if constexpr (S >= 3) {
-                    // ghost state: velocity reflected to the tangential part
-                    // (expliciteuler.cpp:25-58); the flux is G(ghost, ghost)
-                    const double bx = a.bnormal[cell * D + 0], by = a.bnormal[cell * D + 1];
-                    const double rho = uj[0], vx = uj[1] / rho, vy = uj[2] / rho;
-                    const double vn = bx * vx + by * vy;
-                    const double tx = vx - vn * bx, ty = vy - vn * by;
+                    if (wall) {
+                        // ghost state: velocity reflected to the tangential
+                        // part (expliciteuler.cpp:25-58)
+                        const double bx = a.bnormal[cell * D + 0], by = a.bnormal[cell * D + 1];
+                        const double rho = uj[0], vx = uj[1] / rho, vy = uj[2] / rho;
+                        const double vn = bx * vx + by * vy;
+                        const double tx = vx - vn * bx, ty = vy - vn * by;
 #pragma unroll
-                    for (int s = 0; s < S; ++s) v[s] = uj[s];
-                    if constexpr (S == 4) {
-                        if (bc == UQK_BC_NOSLIP) v[3] += rho * 0.5 * ((tx * tx + ty * ty) - (vx * vx + vy * vy));
+                        for (int s = 0; s < S; ++s) gj[s] = uj[s];
+                        if constexpr (S == 4) {
+                            if (bc == UQK_BC_NOSLIP) gj[3] += rho * 0.5 * ((tx * tx + ty * ty) - (vx * vx + vy * vy));
+                        }
+                        gj[1] = rho * tx;
+                        gj[2] = rho * ty;
                     }
-                    v[1] = rho * tx;
-                    v[2] = rho * ty;
-#pragma unroll
-                    for (int s = 0; s < S; ++s) uj[s] = v[s];
-                } else {
-#pragma unroll
-                    for (int s = 0; s < S; ++s) v[s] = uj[s];
                 }
-            } else if (nb == a.ncell) {
-                // the reference reads the never-written uQ[nCells] (zero)
+                for (int e = 0; e < E; ++e) {
+                    const int nb = a.nbr[cell * E + e];
+                    const double* nu = a.unit_normal + (cell * E + e) * D;
+                    const double* nv = a.normal + (cell * E + e) * D;
+                    const double nux = nu[0], nuy = D > 1 ? nu[1] : 0.0, nx = nv[0], ny = D > 1 ? nv[1] : 0.0;
+                    double l[S], r[S], G[S];
+                    if (wall && nb == a.ncell) {
+                        // the flux is G(ghost, ghost)
 #pragma unroll
-                for (int s = 0; s < S; ++s) v[s] = 0.0;
-            } else {
+                        for (int s = 0; s < S; ++s) l[s] = r[s] = S >= 3 ? gj[s] : uj[s];
+                    } else if (nb == a.ncell) {
+                        // the reference reads the never-written uQ[nCells] (zero)
 #pragma unroll
-                for (int s = 0; s < S; ++s) v[s] = a.uq[((long)nb * Q + k) * S + s];
+                        for (int s = 0; s < S; ++s) {
+                            l[s] = uj[s];
+                            r[s] = 0.0;
+                        }
+                    } else {
+#pragma unroll
+                        for (int s = 0; s < S; ++s) {
+                            l[s] = uj[s];
+                            r[s] = a.uq[((long)nb * Q + k) * S + s];
+                        }
+                    }
+                    point_flux<S>(a.p, l, r, nux, nuy, nx, ny, G);
+#pragma unroll
+                    for (int s = 0; s < S; ++s) Gsum[s] += G[s];
+                }
+                if (fric) {
+                    // bed friction S_s = -cf |v| q_s / h (s = 1, 2) at this point
+                    const double cf =
+                        a.p.friction * (1.0 + (a.p.friction_dim >= 0 ? a.xi[k * a.d + a.p.friction_dim] : 0.0));
+                    const double sp = sqrt(uj[1] * uj[1] + uj[2] * uj[2]) / uj[0];
+#pragma unroll
+                    for (int s = 1; s < S; ++s) Gsum[s] += area * (cf * sp * uj[s] / uj[0]);
+                }
+#pragma unroll
+                for (int s = 0; s < S; ++s) Gs[(slot * Q + k) * S + s] = Gsum[s];
             }
-            point_flux<S>(a.p, uj, v, nux, nuy, nx, ny, G);
-#pragma unroll
-            for (int s = 0; s < S; ++s) Gs[t * S + s] = G[s];
         }
         wave_sync();
-        // sum over the edges in edge order, in place into the first edge's slots
-        for (int t = lane; t < Q * S; t += 64) {
-            double sum = 0.0;
-            for (int e = 0; e < E; ++e) sum += Gs[e * Q * S + t];
-            Gs[t] = sum;
-        }
-        wave_sync();
-        // projection rhs(s, i) = sum_k Gsum(s, k) phiW(k, i) as FP64 MFMA
-        // tiles of (S x Q)(Q x M) (lane l: row/column l % 16, k = 4 ks + l/16),
-        // into the edge-flux buffer past the summed fluxes
-        double* rhsb = Gs + Q * S;
-        {
-            const int r16 = lane & 15, kq4 = lane >> 4;
-            const int ks_n = (Q + 3) >> 2;
-            for (int ti = 0; 16 * ti < M; ++ti) {
-                const int ic = 16 * ti + r16;
-                f64x4 pacc = {0.0, 0.0, 0.0, 0.0};
-                for (int ks = 0; ks < ks_n; ++ks) {
-                    const int k = 4 * ks + kq4;
-                    const double av = (r16 < S && k < Q) ? Gs[k * S + r16] : 0.0;
-                    const double bv = (ic < M && k < Q) ? phiw[k * M + ic] : 0.0;
-                    pacc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, pacc, 0, 0, 0);
-                }
+        // projection and update: D = A B with A(r, k) = G(cell slot r / S,
+        // state r % S, point k) and B(k, i) = phiW(k, i), 16 x 16 x 4 FP64
+        // MFMA steps (lane l: row/column l % 16, k = 4 ks + l / 16); element e
+        // of lane l is D(4 e + l / 16, l % 16)
+        const int r16 = lane & 15, kq4 = lane >> 4;
+        const int ks_n = (Q + 3) >> 2;
+        const int ar_slot = r16 / S, ar_s = r16 - ar_slot * S;
+        const long ar_cell = cell0 + ar_slot;
+        const bool ar_on = ar_slot < CPW && ar_cell < a.ncell && a.bc[ar_cell] != UQK_BC_DIRICHLET;
+        for (int ti = 0; 16 * ti < M; ++ti) {
+            const int ic = 16 * ti + r16;
+            f64x4 pacc = {0.0, 0.0, 0.0, 0.0};
+            for (int ks = 0; ks < ks_n; ++ks) {
+                const int k = 4 * ks + kq4;
+                const double av = (ar_on && k < Q) ? Gs[(ar_slot * Q + k) * S + ar_s] : 0.0;
+                const double bv = (ic < M && k < Q) ? phiw[k * M + ic] : 0.0;
+                pacc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, pacc, 0, 0, 0);
+            }
+            if (ic < M) {
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int sr = 4 * e + kq4;  // element e at (4e + l/16, l%16)
-                    if (sr < S && ic < M) rhsb[ic * S + sr] = pacc[e];
-                }
-            }
-        }
-        wave_sync();
-        const double f = dt / a.area[cell];
-        const bool fric = (S == 3 && a.p.problem == UQK_SHALLOW_WATER2D_PROBLEM && a.p.friction > 0.0);
-        for (int e = lane; e < SM; e += 64) {
-            int i = e / S, s = e - i * S;
-            double rhs = 0.0, src = 0.0;
-            if (i < nt) {
-                rhs = rhsb[e];
-                if (fric && s > 0) {
-                    for (int k = 0; k < Q; ++k) {
-                        const double* q = uqj + k * S;
-                        double cf = a.p.friction * (1.0 + (a.p.friction_dim >= 0 ? a.xi[k * a.d + a.p.friction_dim] : 0.0));
-                        double sp = sqrt(q[1] * q[1] + q[2] * q[2]) / q[0];
-                        src += -cf * sp * q[s] / q[0] * phiw[k * M + i];
+                    const int row = 4 * e + kq4;
+                    const int cs = row / S, st = row - cs * S;
+                    const long c = cell0 + cs;
+                    if (cs < CPW && c < a.ncell) {
+                        const int nt = a.nt_cell ? a.nt_cell[c] : M;
+                        const int bcc = a.bc[c];
+                        const int f = ic * S + st;
+                        const double us = a.u[c * SM + f];  // reconstructed moments
+                        double* un = a.u_out + c * SM;       // in: previous moments, out: new moments
+                        const double old = un[f];
+                        double v = us;
+                        if (bcc != UQK_BC_DIRICHLET && ic < nt) v = us - (dt / a.area[c]) * pacc[e];
+                        un[f] = v;
+                        if (f == 0) res += fabs(v - old) * a.area[c];
                     }
                 }
             }
-            double v = us[e] - f * rhs;
-            if (fric) v += dt * src;
-            un[e] = v;
-            if (e == 0) res += fabs(v - old00) * a.area[cell];
         }
         wave_sync();
     }
@@ -378,7 +400,7 @@ __global__ void __launch_bounds__(kSweepThreads) step_end_kernel(RunState* rs, d
         rs->nfail = nfail;
         rs->newton += (long long)counters[2];
         rs->gradients += (long long)counters[3];
-        counters[0] = counters[1] = counters[2] = counters[3] = 0ull;
+        counters[0] = counters[1] = counters[2] = counters[3] = counters[4] = 0ull;
         scalars[0] = 1e10;
         rs->active = (t < rs->t_end && (rs->max_steps < 0 || steps < rs->max_steps) && nfail == 0) ? 1 : 0;
     }
@@ -453,9 +475,10 @@ hipError_t launch_dt(int kind, const SweepArgs& a, hipStream_t st) {
 
 template <int KIND> static void go_advance(const SweepArgs& a, hipStream_t st) {
     constexpr int S = KindInfo<KIND>::S;
-    size_t shmem = sizeof(double) * (kSweepThreads / 64) * advance_wave_doubles(S, a.nedge, a.Q, a.M);
+    size_t shmem = sizeof(double) * (kSweepThreads / 64) * advance_wave_doubles(S, a.Q);
     SweepArgs b = a;
-    b.table_lds = a.Q * a.M <= 2048;  // PhiW up to 16 KB goes to LDS
+    // PhiW goes to LDS while a workgroup stays within 64 KB
+    b.table_lds = shmem + sizeof(double) * a.Q * a.M <= 65536;
     if (b.table_lds) shmem += sizeof(double) * a.Q * a.M;
     hipLaunchKernelGGL(advance_kernel<KIND>, dim3(kPartialBlocks), dim3(kSweepThreads), shmem, st, b);
 }

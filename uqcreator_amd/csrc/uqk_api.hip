@@ -120,12 +120,15 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     if (!counters) counters = ctx->d_counters;
     // (in a replayed run the previous step's step_end_kernel already cleared
     // the counters and set dt's start value: prezeroed)
-    if (!prezeroed) UQK_HIP(ctx, uqk::launch_fill_u64(counters, 4, ctx->stream));  // d_counters[4..7]: warm start
+    // [0] queue head, [1] failed cells, [2] Newton steps, [3] gradients, [4] cells that took the learned start
+    if (!prezeroed) UQK_HIP(ctx, uqk::launch_fill_u64(counters, 5, ctx->stream));
     if (n <= 0) return 0;
-    if (ctx->warm && !nt) {
-        int rc = uqk::enqueue_warm_start(ctx, n, u, lam, active);
-        if (rc) return rc;
-    }
+    // learned warm start (uniform orders): the network's prediction; the wave
+    // kernel selects the start per cell itself, the other kernels after a
+    // selection pass
+    const double* lam_alt = nullptr;
+    if (ctx->warm && !nt)
+        if (int rc = uqk::enqueue_warm_predict(ctx, n, u, &lam_alt)) return rc;
     uqk::NewtonArgs a{};
     a.c = ctx->c;
     a.M = ctx->M;
@@ -142,6 +145,7 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     a.nfail = counters + 1;
     a.prec = ctx->precision;
     a.active = active;
+    a.took = counters + 4;
     const bool want_dt = dt_out && dtp && dx_param && uq_out && u_out && ctx->c.max_iter != 1;
     if (want_dt) {
         // the start value of ComputeDt's MIN (momentsolver.cpp:127); a path
@@ -160,6 +164,7 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         const bool fuse = uq_out && u_out && ctx->c.max_iter != 1;
         a.uq_out = fuse ? uq_out : nullptr;
         a.u_out = fuse ? u_out : nullptr;
+        a.lam_alt = lam_alt;
         if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev0, ctx->stream));
         if (uqk::newton_wave_launch(ctx->kind, ctx->M, ctx->Q, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream, &e,
                                     ctx->shape)) {
@@ -172,7 +177,10 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
             return 0;
         }
         if (ctx->kernel == UQK_KERNEL_WAVE) return fail(ctx, -3, "no specialised closure kernel for this shape");
+        a.lam_alt = nullptr;
     }
+    if (lam_alt)
+        if (int rc = uqk::enqueue_warm_select(ctx, n, u, lam, lam_alt, active)) return rc;
     const bool block_ok = uqk::newton_block_fits(ctx->S, ctx->M, ctx->Q, (int)ctx->max_lds);
     if (ctx->kernel == UQK_KERNEL_BLOCK || (ctx->kernel == UQK_KERNEL_AUTO && ctx->S * ctx->M >= 64 && block_ok)) {
         if (!block_ok) return fail(ctx, -3, "closure too large for the workgroup kernel");
@@ -345,14 +353,14 @@ int resolve_scratch(uqk_ctx* ctx) {
 
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
-// Layout of one uqk_closure_solve_host_noalloc call in `scratch`: the four
+// Layout of one uqk_closure_solve_host_noalloc call in `scratch`: the five
 // closure counters, then u, lambda, status, iters and (if given) nTotal, each
 // on a 256-byte boundary.  Returns the bytes used.
 size_t noalloc_layout(int64_t n, int S, int M, bool with_nt, size_t off[6]) {
     const size_t cb = sizeof(double) * (size_t)n * S * M;
     const size_t ib = sizeof(int32_t) * (size_t)n;
     size_t at = 0;
-    off[0] = at, at += align256(4 * sizeof(unsigned long long));
+    off[0] = at, at += align256(5 * sizeof(unsigned long long));
     off[1] = at, at += align256(cb);
     off[2] = at, at += align256(cb);
     off[3] = at, at += align256(ib);
@@ -733,7 +741,7 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
     UQK_HIP(ctx, hipMemcpy(ctx->d_run, ctx->h_run, sizeof(uqk::RunState), hipMemcpyHostToDevice));
     // what each step's step_end_kernel leaves for the next one: zero closure
     // counters and dt's start value (momentsolver.cpp:127)
-    UQK_HIP(ctx, uqk::launch_fill_u64(ctx->d_counters, 4, ctx->stream));
+    UQK_HIP(ctx, uqk::launch_fill_u64(ctx->d_counters, 5, ctx->stream));
     UQK_HIP(ctx, uqk::launch_fill_f64(ctx->d_scalars, 1, 1e10, ctx->stream));
     const int* active = &ctx->d_run->active;
     auto enqueue = [&]() -> int {

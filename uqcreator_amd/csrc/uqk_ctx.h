@@ -74,9 +74,9 @@ inline void invalidate_graph(uqk_ctx* ctx) {
 // Grows the attached network's buffers for a warm start over ncell cells
 // (outside graph capture, which cannot allocate).
 int prepare_warm_start(uqk_ctx* ctx, long ncell);
-// learned warm start (mlp.hip): predict with the network attached to ctx and
-// keep per cell the start with the smaller ||grad L||; enqueue only.
-int enqueue_warm_start(uqk_ctx* ctx, long ncell, const double* u, double* lambda, const int* active);
+// learned warm start (mlp.hip): the attached network's starting duals for the
+// moments u into its own buffer (*lam_alt; null without a network); enqueue only.
+int enqueue_warm_predict(uqk_ctx* ctx, long ncell, const double* u, const double** lam_alt);
 int enqueue_warm_select(uqk_ctx* ctx, long ncell, const double* u, double* lambda, const double* lambda_alt,
                         const int* active);
 }  // namespace uqk

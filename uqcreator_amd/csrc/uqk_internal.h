@@ -37,6 +37,12 @@ struct NewtonArgs {
     ProblemParams dtp;
     const double* dx_param;
     unsigned long long* dt_min;
+    // learned warm start, wave kernel: when lam_alt is not null, a cell whose
+    // lambda fails the first convergence test also evaluates ||grad L|| at
+    // lam_alt and starts Newton from whichever is smaller (non-finite never
+    // wins); *took += cells that started from lam_alt
+    const double* lam_alt;
+    unsigned long long* took;
 };
 
 // Device-side state of a multi-step run replayed from a captured graph.  The
