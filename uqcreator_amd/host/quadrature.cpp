@@ -19,7 +19,10 @@
 
 namespace uqk_host {
 
-// Symmetric tridiagonal eigenproblem by implicit QL with Wilkinson shifts.
+// Symmetric tridiagonal eigenproblem by implicit QL with Wilkinson shifts:
+// the published tql2 method (Bowdler, Martin, Reinsch and Wilkinson, Numer.
+// Math. 11 (1968) 293-306; EISPACK TQL2), the method mathtools.cpp:41-104
+// also follows.
 // d: diagonal (eigenvalues on return), e: sub-diagonal e[0..n-2],
 // z: n x n row-major, returns eigenvectors in columns.
 static bool tridiag_ql(std::vector<double>& d, std::vector<double>& e, std::vector<double>& z, int n) {
