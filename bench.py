@@ -51,10 +51,15 @@ C3_CFL = 0.4
 
 
 def flop_model(S, M, Q):
-    """Algorithmic flops of one Newton step (symmetric Hessian Q n (n+1),
-    Cholesky + solve n^3/3 + 2 n^2) and of one gradient evaluation (4 Q S M)."""
+    """Algorithmic flops of one factored Newton step (symmetric Hessian
+    Q n (n+1), Cholesky + solve n^3/3 + 2 n^2), of the inverse factor the
+    mixed closure keeps after it (triangular inverse n^3/3), of one step on a
+    kept inverse (two applications of V V^T, 4 n^2, and the FP64
+    Hessian-vector product 4 Q S M + 2 Q S^2) and of one gradient evaluation
+    (4 Q S M)."""
     n = S * M
-    return dict(F_H=Q * n * (n + 1), F_C=n ** 3 / 3.0 + 2.0 * n * n, F_g=4.0 * Q * S * M)
+    return dict(F_H=Q * n * (n + 1), F_C=n ** 3 / 3.0 + 2.0 * n * n, F_inv=n ** 3 / 3.0,
+                F_lag=4.0 * n * n + 4.0 * Q * S * M + 2.0 * Q * S * S, F_g=4.0 * Q * S * M)
 
 
 def advance_bytes_per_cell(S, M, Q, edges, dim):
@@ -75,12 +80,18 @@ def cpu_threads():
     return max(1, min(16, n))
 
 
-def closure_roofline(kernel, kernel_ms, newton, grads, fm, precision_mixed):
+def closure_roofline(kernel, kernel_ms, newton, grads, fm, precision_mixed, factors=None, lagged=0.0):
     """Roofline of a closure kernel: algorithmic flops per launch over its
     HIP-event time; the peak is time-weighted over the units the flops run
-    on (Hessian on FP64 or FP32 MFMA, the rest FP64)."""
-    f_hess = newton * fm["F_H"]
-    f_fp64 = newton * fm["F_C"] + grads * fm["F_g"]
+    on (Hessian on FP64 or FP32 MFMA, the rest FP64).  factors / lagged:
+    factored Newton steps and steps on a kept inverse (mixed closure); by
+    default every Newton step is factored."""
+    if factors is None:
+        factors = newton
+    f_hess = factors * fm["F_H"]
+    f_fp64 = factors * fm["F_C"] + grads * fm["F_g"]
+    if lagged or (precision_mixed and factors != newton):
+        f_fp64 += factors * fm["F_inv"] + lagged * fm["F_lag"]
     flops = f_hess + f_fp64
     hess_peak = FP32_PEAK_TFLOPS if precision_mixed else FP64_PEAK_TFLOPS
     t_min = f_hess / (hess_peak * 1e12) + f_fp64 / (FP64_PEAK_TFLOPS * 1e12)
@@ -89,6 +100,7 @@ def closure_roofline(kernel, kernel_ms, newton, grads, fm, precision_mixed):
     return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
             "traffic": None, "kernel": kernel, "kernel_ms": kernel_ms, "flops_per_launch": flops,
             "hessian_flops_per_launch": f_hess, "newton_steps_per_launch": newton,
+            "factorizations_per_launch": factors, "lagged_steps_per_launch": lagged,
             "gradient_evals_per_launch": grads, "flop_model": fm}
 
 
@@ -149,6 +161,7 @@ class C3:
         elapsed = time.perf_counter() - t0
         assert s.steps == args.steps, (s.steps, args.steps)
         newton, grads = cl.last_closure_stats()
+        factors, lagged = cl.last_closure_factorizations()
         mv = s.mean_var()
         t_end = s.t
         # the same K steps again without the graph: HIP events around every
@@ -170,9 +183,12 @@ class C3:
         st = s.status.download()
         fm = flop_model(4, self.b.M, self.b.Q)
         S, M, Q = 4, self.b.M, self.b.Q
-        kname = "newton_wave_kernel<2,10,25,%d>" % precision
+        # the mixed time loop with kept inverses runs instantiation 2 and the
+        # inverse kernel, both inside the closure's timing window
+        kname = ("newton_wave_kernel<2,10,25,2> + factor_inverse_kernel<40>" if lagged > 0 else
+                 "newton_wave_kernel<2,10,25,%d>" % precision)
         roof_c = closure_roofline(kname, closure_ms / timed_steps, newton / args.steps, grads / args.steps, fm,
-                                  precision == native.MIXED)
+                                  precision == native.MIXED, factors / args.steps, lagged / args.steps)
         bpc = advance_bytes_per_cell(S, M, Q, self.m.nbr.shape[1], 2)
         adv_ms = advance_ms / timed_steps
         adv_gbs = bpc * self.n / (adv_ms * 1e-3) / 1e9
@@ -184,6 +200,8 @@ class C3:
             "ms_per_step": 1e3 * elapsed / args.steps,
             "t_end": t_end,
             "newton_steps_per_cell_step": newton / (self.n * args.steps),
+            "factorizations_per_cell_step": factors / (self.n * args.steps),
+            "lagged_steps_per_cell_step": lagged / (self.n * args.steps),
             "gradient_evals_per_cell_step": grads / (self.n * args.steps),
             "newton_histogram_at_step": hist,
             "status_counts": np.bincount(st, minlength=5).tolist(),
@@ -266,7 +284,7 @@ def c2_leg(native, np, b, u, lam0, lam_star, args, precision, warm=None):
         lam.copy_from(lam0)
         cl.solve(u, lam, st, it)
     native.synchronize()
-    kernel_ms, newton, grads = [], 0, 0
+    kernel_ms, newton, grads, factors, lagged = [], 0, 0, 0, 0
     t0 = time.perf_counter()
     for _ in range(args.c2_steps):
         lam.copy_from(lam0)
@@ -274,8 +292,11 @@ def c2_leg(native, np, b, u, lam0, lam_star, args, precision, warm=None):
         cl.solve(u, lam, st, it)
         kernel_ms.append(cl.last_closure_ms())
         a, g = cl.last_closure_stats()
+        f, lg = cl.last_closure_factorizations()
         newton += a
         grads += g
+        factors += f
+        lagged += lg
     native.synchronize()
     elapsed = time.perf_counter() - t0
     status = st.download()
@@ -289,11 +310,14 @@ def c2_leg(native, np, b, u, lam0, lam_star, args, precision, warm=None):
         "value": args.c2_cells / (elapsed / args.c2_steps),
         "ms_per_step": 1e3 * elapsed / args.c2_steps,
         "newton_steps_per_cell": newton / args.c2_steps / args.c2_cells,
+        "factorizations_per_cell": factors / args.c2_steps / args.c2_cells,
+        "lagged_steps_per_cell": lagged / args.c2_steps / args.c2_cells,
         "gradient_evals_per_cell": grads / args.c2_steps / args.c2_cells,
         "status_counts": np.bincount(status, minlength=5).tolist(),
         "lambda_rel_err_vs_known": float((np.abs(lam_h - lam_star) / scale).max()),
         "roofline": closure_roofline(kname, float(np.mean(kernel_ms)), newton / args.c2_steps,
-                                     grads / args.c2_steps, fm, precision == native.MIXED),
+                                     grads / args.c2_steps, fm, precision == native.MIXED, factors / args.c2_steps,
+                                     lagged / args.c2_steps),
     }
     if net is not None:
         net.detach()
@@ -435,6 +459,8 @@ def main():
     ref = mvs.get("fp64")
     for name in legs:
         legs[name]["mean_var_rel_l2_vs_fp64"] = rel_l2(mvs[name], ref, c3.m.area) if ref is not None else None
+        legs[name]["mean_var_max_abs_diff_vs_fp64"] = float(np.abs(mvs[name] - ref).max()) if ref is not None else None
+        legs[name]["mean_var_sum"] = float(mvs[name].sum())
     head_name = "mixed" if "mixed" in legs else legs_wanted[-1]
     head = legs[head_name]
     prec_head = native.FP64 if head_name == "fp64" else native.MIXED

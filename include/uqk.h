@@ -244,6 +244,25 @@ double uqk_last_closure_ms(const uqk_ctx* ctx);
  * gradient evaluations (line-search trials included) over all cells; after
  * uqk_run_steps, the totals over the run's steps. */
 int uqk_last_closure_stats(const uqk_ctx* ctx, int64_t* newton_steps, int64_t* gradient_evaluations);
+/* Reuse of the factorisation in the mixed closure (no counterpart in the
+ * reference; on by default for UQK_MIXED contexts; the time loop, wave
+ * kernel, uniform orders).  After a cell's FP64 Cholesky of its
+ * (FP32-assembled) Hessian in one step, the library computes the inverse of
+ * the factored matrix (FP64, kept in FP32); the cell's Newton steps in later
+ * time steps take d = P g refined once with the FP64 Hessian-vector product
+ * (d += P (g - H d)) and keep it when the full step lowers ||grad L|| (the
+ * FP64 accept test); otherwise, and after a contraction clearly worse than a
+ * damped Newton step's (above 1.2 (1 - alpha) + 1e-3), the cell's current
+ * Hessian is factored and the reference Newton step runs.  Memory per cell:
+ * n (n + 1) floats and n (n + 3) / 2 doubles.  Single closure calls run the
+ * reference Newton.  Results match the plain mixed closure to the Newton
+ * tolerance, not bit for bit.  uqk_invalidate_factors forgets the kept
+ * inverses (after replacing the duals wholesale). */
+int uqk_set_factor_reuse(uqk_ctx* ctx, int32_t on);
+int uqk_invalidate_factors(uqk_ctx* ctx);
+/* Cholesky factorisations and steps on a kept inverse of the last closure
+ * call (after uqk_run_steps: of the run). */
+int uqk_last_closure_factorizations(const uqk_ctx* ctx, int64_t* factorizations, int64_t* reuse_steps);
 /* Per-step kernel timing (no counterpart in the reference): while on, every
  * step that uqk_step, or uqk_run_steps without a graph, enqueues brackets its
  * closure kernel and its advance kernel with HIP events on the context's

@@ -232,15 +232,18 @@ def test_mixed_precision_closure(oracle, kind, d, deg, q):
     c = oracle.closure(oracle.EULER1D if kind == native.EULER1D else kind_o, **okw)
     lam_o = lam0.copy()
     st_o, it_o = oracle.solve_closure(c, b.phi, b.phiw, u, lam_o)
+    scale = np.abs(lam_star).max(axis=1, keepdims=True)
     cl = native.GpuClosure(kind, b, precision=native.MIXED, **gkw)
     nfail, lam, st, it = _run_gpu(cl, u, lam0)
     assert cl.last_kernel() == native.KERNEL_WAVE
     assert nfail == 0 and (st == 0).all() and (st_o == 0).all()
-    scale = np.abs(lam_star).max(axis=1, keepdims=True)
     assert (np.abs(lam - lam_o) / scale).max() < 1e-8
     assert (np.abs(lam - lam_star) / scale).max() < 1e-7
     # the FP32 Hessian changes the Newton path only slightly
     assert it.mean() <= it_o.mean() + 1.0
+    # a single closure call factors at every Newton step
+    factors, lagged = cl.last_closure_factorizations()
+    assert lagged == 0 and factors == it.sum()
 
 
 def test_autotuned_launch_shape_gives_identical_duals(tmp_path):

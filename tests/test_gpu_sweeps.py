@@ -235,3 +235,40 @@ def test_euler2d_c3_full_grid(oracle, precision, tol):
     err = _rel_l2(mv, mv_o, m.area)
     print("C3 full grid rel L2 E/Var", err, "newton GPU", s.newton_steps, "oracle", res["newton_iters"], flush=True)
     assert err.max() < tol
+
+
+def test_mixed_factor_reuse_time_loop(oracle):
+    """The time loop's mixed closure keeps each cell's inverse Hessian from
+    its last factorisation and takes refined lagged Newton steps on it (FP64
+    accept test): on the NACA0012 O-grid the E/Var fields after 60 steps stay
+    within 1e-6 of the oracle's, as with a factorisation at every step, and
+    far fewer Cholesky factorisations run."""
+    b, m, u0 = workloads.euler2d_c3(128, 32)
+    lam0 = np.zeros_like(u0)
+    lam0[:, 0] = workloads.euler2d_ds(u0[:, 0])
+    steps = 60
+    prob = native.problem(native.EULER2D_PROBLEM, flux=native.FLUX_LAX_FRIEDRICHS, cfl=0.4)
+    c = oracle.closure(oracle.EULER2D, eps=1e-6)
+    uo, lo = u0.copy(), lam0.copy()
+    res = oracle.run(c, oracle.problem(oracle.P_EULER2D, flux=oracle.FLUX_LF, cfl=0.4), _oracle_mesh(oracle, m),
+                     b.phi, b.phiw, b.xi, uo, lo, 1e3, steps)
+    assert res["rc"] == 0
+    mv_o = oracle.mean_var(c, b.phi, b.phiw, lo)
+    got = {}
+    for reuse in (False, True):
+        print("case: mixed time loop 128x32, %d steps, factor reuse %s" % (steps, reuse), flush=True)
+        cl = native.GpuClosure(native.EULER2D, b, epsilon=1e-6, precision=native.MIXED)
+        cl.set_factor_reuse(reuse)
+        s = solver.GpuMomentSolver(cl, prob, m)
+        s.set_state(u0, lam0)
+        s.run_device(1e3, max_steps=steps)
+        assert s.steps == steps
+        factors, lagged = cl.last_closure_factorizations()
+        err = _rel_l2(s.mean_var(), mv_o, m.area)
+        print("reuse", reuse, "rel L2 E/Var", err, "Newton steps", s.newton_steps, "factorisations", factors,
+              "lagged steps", lagged, flush=True)
+        assert err.max() < 1e-6
+        got[reuse] = (factors, lagged, s.newton_steps)
+    assert got[False][1] == 0 and got[False][0] == got[False][2]
+    assert got[True][0] + got[True][1] <= got[True][2]  # (a rejected lagged step counts as a Newton step)
+    assert got[True][0] < 0.5 * got[False][0]

@@ -39,7 +39,8 @@ EXTRA_FLAGS = {
     # the one-wavefront closure is a chain of dependent steps (pivot
     # broadcasts, Newton steps); LLVM's iterative ILP scheduler orders it
     # better than the default occupancy-driven one (about 3% more throughput
-    # for the mixed closure, the same for FP64)
+    # for the mixed closure, the same for FP64); closure_wave_reuse.hip keeps
+    # the default one, which spills less there
     "closure_wave.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
 }
 

@@ -400,7 +400,10 @@ __global__ void __launch_bounds__(kSweepThreads) step_end_kernel(RunState* rs, d
         rs->nfail = nfail;
         rs->newton += (long long)counters[2];
         rs->gradients += (long long)counters[3];
-        counters[0] = counters[1] = counters[2] = counters[3] = counters[4] = 0ull;
+        rs->factorizations += (long long)counters[5];
+        rs->reuse_steps += (long long)counters[6];
+#pragma unroll
+        for (int i = 0; i < kClosureCounters; ++i) counters[i] = 0ull;
         scalars[0] = 1e10;
         rs->active = (t < rs->t_end && (rs->max_steps < 0 || steps < rs->max_steps) && nfail == 0) ? 1 : 0;
     }

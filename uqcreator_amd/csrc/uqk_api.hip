@@ -102,8 +102,36 @@ int check_mesh(uqk_ctx* ctx, const uqk_mesh* m) {
     return 0;
 }
 
+// Kept inverse factors of the mixed wave kernel across time steps: grows
+// the per-cell slots (flags cleared) to n cells.  Allocates only outside
+// graph capture.
+int ensure_factor_buffers(uqk_ctx* ctx, long n_cells) {
+    const long n = (long)ctx->S * ctx->M;
+    const long nn = n * (n + 1);  // n x n FP32, row stride n + 1, per cell
+    const long nu = n * (n + 1) / 2 + n;  // packed U and 1/U_rr, FP64, per cell
+    if (ctx->pinv_cell_cap >= n_cells) return 0;
+    if (ctx->capturing) return fail(ctx, -1, "kept inverses: buffers must exist before graph capture");
+    hipFree(ctx->pinv_cell);
+    hipFree(ctx->ufac_cell);
+    hipFree(ctx->pinv_ok);
+    hipFree(ctx->pinv_list);
+    ctx->pinv_cell = nullptr;
+    ctx->ufac_cell = nullptr;
+    ctx->pinv_ok = nullptr;
+    ctx->pinv_list = nullptr;
+    ctx->pinv_cell_cap = 0;
+    UQK_HIP(ctx, hipMalloc(&ctx->pinv_cell, sizeof(float) * (size_t)nn * n_cells));
+    UQK_HIP(ctx, hipMalloc(&ctx->ufac_cell, sizeof(double) * (size_t)nu * n_cells));
+    UQK_HIP(ctx, hipMalloc(&ctx->pinv_ok, sizeof(int) * (size_t)n_cells));
+    UQK_HIP(ctx, hipMalloc(&ctx->pinv_list, sizeof(int) * (size_t)n_cells));
+    UQK_HIP(ctx, hipMemsetAsync(ctx->pinv_ok, 0, sizeof(int) * (size_t)n_cells, ctx->stream));
+    ctx->pinv_cell_cap = n_cells;
+    return 0;
+}
+
 // Launch the Newton closure kernel; leaves the failed-cell count in counters[1]
-// (counters: the context's d_counters unless the caller brings its own four).
+// (counters: the context's d_counters unless the caller brings its own
+// kClosureCounters).
 // When uq_out/u_out are given and the specialised or the workgroup kernel runs
 // (not one-shot), it also writes uQ and the reconstructed moments and *fused
 // is set.
@@ -113,7 +141,7 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
                    int32_t* iters, const int* active = nullptr, double* uq_out = nullptr, double* u_out = nullptr,
                    bool* fused = nullptr, double* dt_out = nullptr, const uqk::ProblemParams* dtp = nullptr,
                    const double* dx_param = nullptr, bool* fused_dt = nullptr,
-                   unsigned long long* counters = nullptr, bool prezeroed = false) {
+                   unsigned long long* counters = nullptr, bool prezeroed = false, bool factors_per_cell = false) {
     if (fused) *fused = false;
     if (fused_dt) *fused_dt = false;
     ctx->timed = false;
@@ -121,7 +149,7 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     // (in a replayed run the previous step's step_end_kernel already cleared
     // the counters and set dt's start value: prezeroed)
     // [0] queue head, [1] failed cells, [2] Newton steps, [3] gradients, [4] cells that took the learned start
-    if (!prezeroed) UQK_HIP(ctx, uqk::launch_fill_u64(counters, 5, ctx->stream));
+    if (!prezeroed) UQK_HIP(ctx, uqk::launch_fill_u64(counters, uqk::kClosureCounters, ctx->stream));
     if (n <= 0) return 0;
     // learned warm start (uniform orders): the network's prediction; the wave
     // kernel selects the start per cell itself, the other kernels after a
@@ -146,6 +174,23 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     a.prec = ctx->precision;
     a.active = active;
     a.took = counters + 4;
+    a.nfactor = counters + 5;
+    // mixed precision: kept inverses (per cell in the time loop, per wave for
+    // a single closure call); the buffers exist before a graph is captured
+    // (the time loop only: each cell's inverse is computed after the launch
+    // that factored, for the next steps)
+    if (ctx->precision == UQK_MIXED && ctx->factor_reuse && !nt && ctx->c.max_iter != 1 && factors_per_cell &&
+        (ctx->kernel == UQK_KERNEL_AUTO || ctx->kernel == UQK_KERNEL_WAVE) &&
+        uqk::newton_wave_reuse_fits(ctx->kind, ctx->M, ctx->Q, (int)ctx->max_lds)) {
+        if (int rc = ensure_factor_buffers(ctx, n)) return rc;
+        a.pinv_mode = 1;
+        a.pinv = ctx->pinv_cell;
+        a.ufac = ctx->ufac_cell;
+        a.pinv_ok = ctx->pinv_ok;
+        a.plist = ctx->pinv_list;
+        a.plist_count = counters + 7;
+        a.pinv_slots = ctx->pinv_cell_cap;
+    }
     const bool want_dt = dt_out && dtp && dx_param && uq_out && u_out && ctx->c.max_iter != 1;
     if (want_dt) {
         // the start value of ComputeDt's MIN (momentsolver.cpp:127); a path
@@ -169,6 +214,14 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         if (uqk::newton_wave_launch(ctx->kind, ctx->M, ctx->Q, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream, &e,
                                     ctx->shape)) {
             UQK_HIP(ctx, e);
+            if (a.pinv_mode == 1) {
+                // the inverses of this launch's new factors, for the next steps
+                if (!uqk::factor_inverse_launch(ctx->kind, ctx->M, ctx->Q, n, counters + 7, ctx->pinv_list,
+                                                ctx->pinv_ok, ctx->ufac_cell, ctx->pinv_cell, active, ctx->num_cu,
+                                                ctx->stream, &e))
+                    return fail(ctx, -3, "no factor inverse kernel for this shape");
+                UQK_HIP(ctx, e);
+            }
             if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev1, ctx->stream));
             ctx->last_kernel = UQK_KERNEL_WAVE;
             ctx->timed = ev;
@@ -266,7 +319,7 @@ int enqueue_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, in
     }
     int rc = launch_closure(ctx, n, nt, u, lam, status, iters, active, refining ? nullptr : uq,
                             refining ? nullptr : u_scratch, &fused, ctx->d_scalars, &pp, mesh->dx_param, &fused_dt,
-                            nullptr, in_run);
+                            nullptr, in_run, true);
     ctx->cev0 = ctx->cev1 = nullptr;
     if (rc) return rc;
     uqk::SweepArgs a = sweep_args(ctx);
@@ -325,7 +378,7 @@ int collect_step_times(uqk_ctx* ctx) {
 int64_t finish_closure(uqk_ctx* ctx, const unsigned long long* counters = nullptr) {
     if (!counters) counters = ctx->d_counters;
     UQK_HIP(ctx, hipMemcpyAsync(ctx->last_stats, counters + 1, sizeof(ctx->last_stats), hipMemcpyDeviceToHost,
-                                ctx->stream));
+                                ctx->stream));  // counters 1..6
     UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
     ctx->last_closure_ms = 0.0;
@@ -353,14 +406,14 @@ int resolve_scratch(uqk_ctx* ctx) {
 
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
-// Layout of one uqk_closure_solve_host_noalloc call in `scratch`: the five
+// Layout of one uqk_closure_solve_host_noalloc call in `scratch`: the
 // closure counters, then u, lambda, status, iters and (if given) nTotal, each
 // on a 256-byte boundary.  Returns the bytes used.
 size_t noalloc_layout(int64_t n, int S, int M, bool with_nt, size_t off[6]) {
     const size_t cb = sizeof(double) * (size_t)n * S * M;
     const size_t ib = sizeof(int32_t) * (size_t)n;
     size_t at = 0;
-    off[0] = at, at += align256(5 * sizeof(unsigned long long));
+    off[0] = at, at += align256(uqk::kClosureCounters * sizeof(unsigned long long));
     off[1] = at, at += align256(cb);
     off[2] = at, at += align256(cb);
     off[3] = at, at += align256(ib);
@@ -428,7 +481,7 @@ int uqk_create(const uqk_desc* desc, uqk_ctx** out) {
         UQK_HIP(ctx, hipMalloc(&ctx->d_xi, xb));
         UQK_HIP(ctx, hipMemcpy(ctx->d_xi, desc->xi, xb, hipMemcpyHostToDevice));
     }
-    UQK_HIP(ctx, hipMalloc(&ctx->d_counters, 8 * sizeof(unsigned long long)));
+    UQK_HIP(ctx, hipMalloc(&ctx->d_counters, uqk::kClosureCounters * sizeof(unsigned long long)));
     UQK_HIP(ctx, hipMalloc(&ctx->d_partial, sizeof(double) * uqk::sweep_partial_blocks(0)));
     UQK_HIP(ctx, hipMalloc(&ctx->d_scalars, 4 * sizeof(double)));
     UQK_HIP(ctx, hipStreamCreate(&ctx->own_stream));
@@ -448,6 +501,10 @@ int uqk_destroy(uqk_ctx* ctx) {
     hipFree(ctx->d_counters);
     hipFree(ctx->d_partial);
     hipFree(ctx->d_scalars);
+    hipFree(ctx->pinv_cell);
+    hipFree(ctx->ufac_cell);
+    hipFree(ctx->pinv_ok);
+    hipFree(ctx->pinv_list);
     if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
     hipFree(ctx->d_run);
     if (ctx->h_run) hipHostFree(ctx->h_run);
@@ -730,8 +787,11 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
     UQK_HIP(ctx, hipSetDevice(ctx->device));
     if (!ctx->d_run) UQK_HIP(ctx, hipMalloc(&ctx->d_run, sizeof(uqk::RunState)));
     if (!ctx->h_run) UQK_HIP(ctx, hipHostMalloc((void**)&ctx->h_run, sizeof(uqk::RunState), hipHostMallocDefault));
-    // the warm start's buffers must exist before the step is captured
+    // the warm start's and the kept inverses' buffers must exist before the
+    // step is captured
     if (int rc = uqk::prepare_warm_start(ctx, mesh->n_cells)) return rc;
+    if (ctx->precision == UQK_MIXED && ctx->factor_reuse && !nt && mesh->n_cells > 0)
+        if (int rc = ensure_factor_buffers(ctx, mesh->n_cells)) return rc;
     UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *ctx->h_run = uqk::RunState{};
     ctx->h_run->t = t0;
@@ -741,7 +801,7 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
     UQK_HIP(ctx, hipMemcpy(ctx->d_run, ctx->h_run, sizeof(uqk::RunState), hipMemcpyHostToDevice));
     // what each step's step_end_kernel leaves for the next one: zero closure
     // counters and dt's start value (momentsolver.cpp:127)
-    UQK_HIP(ctx, uqk::launch_fill_u64(ctx->d_counters, 5, ctx->stream));
+    UQK_HIP(ctx, uqk::launch_fill_u64(ctx->d_counters, uqk::kClosureCounters, ctx->stream));
     UQK_HIP(ctx, uqk::launch_fill_f64(ctx->d_scalars, 1, 1e10, ctx->stream));
     const int* active = &ctx->d_run->active;
     auto enqueue = [&]() -> int {
@@ -811,6 +871,9 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
     ctx->last_stats[0] = (unsigned long long)rs.nfail;  // uqk_last_closure_stats: the run's totals
     ctx->last_stats[1] = (unsigned long long)rs.newton;
     ctx->last_stats[2] = (unsigned long long)rs.gradients;
+    ctx->last_stats[3] = 0;
+    ctx->last_stats[4] = (unsigned long long)rs.factorizations;
+    ctx->last_stats[5] = (unsigned long long)rs.reuse_steps;
     if (t_out) *t_out = rs.t;
     if (steps_out) *steps_out = rs.steps;
     if (residual_out) *residual_out = rs.residual;
@@ -846,6 +909,30 @@ int uqk_set_kernel_timing(uqk_ctx* ctx, int32_t on) {
     ctx->ktiming = on != 0;
     ctx->k_closure_ms = ctx->k_advance_ms = 0.0;
     ctx->k_steps = 0;
+    return 0;
+}
+
+int uqk_set_factor_reuse(uqk_ctx* ctx, int32_t on) {
+    if (!ctx) return -1;
+    if (ctx->factor_reuse != (on != 0)) uqk::invalidate_graph(ctx);
+    ctx->factor_reuse = on != 0;
+    return 0;
+}
+
+int uqk_invalidate_factors(uqk_ctx* ctx) {
+    if (!ctx) return -1;
+    if (ctx->pinv_ok && ctx->pinv_cell_cap > 0) {
+        UQK_HIP(ctx, hipSetDevice(ctx->device));
+        UQK_HIP(ctx, hipMemsetAsync(ctx->pinv_ok, 0, sizeof(int) * (size_t)ctx->pinv_cell_cap, ctx->stream));
+        UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return 0;
+}
+
+int uqk_last_closure_factorizations(const uqk_ctx* ctx, int64_t* factorizations, int64_t* reuse_steps) {
+    if (!ctx) return -1;
+    if (factorizations) *factorizations = (int64_t)ctx->last_stats[4];
+    if (reuse_steps) *reuse_steps = (int64_t)ctx->last_stats[5];
     return 0;
 }
 

@@ -31,7 +31,9 @@ struct uqk_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_closure_ms = 0.0;
-    unsigned long long last_stats[3] = {0, 0, 0};
+    // the last closure call's (or run's) counters 1..6: failed cells, Newton
+    // steps, gradients, learned starts taken, factorisations, kept-inverse steps
+    unsigned long long last_stats[6] = {0, 0, 0, 0, 0, 0};
     int kernel = UQK_KERNEL_AUTO;
     int last_kernel = UQK_KERNEL_AUTO;
     bool timed = false;  // tev0/tev1 bracket the last closure kernel
@@ -55,6 +57,16 @@ struct uqk_ctx {
     uqk_mlp* warm = nullptr;  // learned warm start run before each closure (not owned)
     uqk::RefineArgs refine{};  // adaptive levels: refine.nlev > 1 turns refinement on in the step
     unsigned char* scratch = nullptr;  // device address of the library's static staging array on `device`
+    // reuse of the factorisation in the mixed wave kernel across time
+    // steps: each cell's kept inverse Hessian (FP32, n x n, row stride n + 1),
+    // its last factor (FP64 packed U, 1/U_rr) and state (0 none, 1 inverse
+    // kept, 2 factor stored, inverse pending)
+    bool factor_reuse = true;
+    float* pinv_cell = nullptr;
+    double* ufac_cell = nullptr;
+    int* pinv_ok = nullptr;
+    int* pinv_list = nullptr;  // cells with a new factor, per launch
+    long pinv_cell_cap = 0;
     std::string err;
 };
 

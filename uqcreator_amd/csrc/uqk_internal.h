@@ -43,6 +43,23 @@ struct NewtonArgs {
     // wins); *took += cells that started from lam_alt
     const double* lam_alt;
     unsigned long long* took;
+    // mixed precision, wave kernel, time loop: reuse of the factorisation
+    // across steps (pinv_mode 1).  A cell whose pinv_ok is 1 holds a kept
+    // inverse Hessian in pinv (FP32, n x n, row stride n + 1) and takes
+    // Newton steps d = P g refined once with the FP64 Hessian-vector
+    // product; a factored step stores its factor in ufac (packed upper U,
+    // then 1/U_rr, FP64), sets pinv_ok to 2 and appends the cell to
+    // plist[(*plist_count)++], and factor_inverse_kernel turns the listed
+    // factors into the next inverses.  nfactor[0] += Cholesky
+    // factorisations, nfactor[1] += steps on a kept inverse.
+    float* pinv;
+    double* ufac;
+    int* pinv_ok;
+    int* plist;
+    unsigned long long* plist_count;
+    int pinv_mode;
+    long pinv_slots;
+    unsigned long long* nfactor;
 };
 
 // Device-side state of a multi-step run replayed from a captured graph.  The
@@ -56,10 +73,18 @@ struct RunState {
     long long newton;
     long long gradients;
     long long nfail;
+    long long factorizations;
+    long long reuse_steps;
     double t_end;
     long long max_steps;  // < 0: no limit
     int active;
 };
+
+// Closure counters of a launch: [0] work-queue head, [1] failed cells, [2]
+// Newton steps, [3] gradient evaluations, [4] cells that started from the
+// learned warm start, [5] Cholesky factorisations, [6] steps on a kept
+// inverse, [7] cells listed for factor_inverse_kernel; all zero at launch.
+constexpr int kClosureCounters = 8;
 
 // Launch shape of the specialised closure kernel; 0 = automatic (as many
 // waves per workgroup as LDS allows, every resident workgroup slot filled).
@@ -78,6 +103,14 @@ inline void set_queue(NewtonArgs& a, long resident_waves) {
 
 bool newton_wave_launch(int kind, int M, int Q, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
                         hipError_t* err, const LaunchShape& shape = LaunchShape());
+// Whether (kind, M, Q) has the mixed wave kernel with kept inverses.
+bool newton_wave_reuse_fits(int kind, int M, int Q, int max_lds);
+// Inverses of the factors the last mixed wave launch stored and listed
+// (list[0 .. *count), flags 2 -> 1), up to ncell of them; false when there
+// is no such kernel for (kind, M, Q).
+bool factor_inverse_launch(int kind, int M, int Q, long ncell, const unsigned long long* count, const int* list,
+                           int* flags, const double* ufac, float* pinv, const int* active, int num_cu,
+                           hipStream_t st, hipError_t* err);
 // Largest waves per workgroup the LDS allows and the resident workgroups per
 // CU at `waves` for the specialised kernel of (kind, M, Q, prec); false when
 // there is none.

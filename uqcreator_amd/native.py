@@ -101,6 +101,9 @@ SIGNATURES = {
     "uqk_last_closure_ms": (C.c_double, [C.c_void_p]),
     "uqk_last_closure_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "uqk_set_kernel_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+    "uqk_set_factor_reuse": (C.c_int, [C.c_void_p, C.c_int32]),
+    "uqk_invalidate_factors": (C.c_int, [C.c_void_p]),
+    "uqk_last_closure_factorizations": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "uqk_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "uqk_synth_euler1d_closure": (C.c_int64, [C.c_int64, C.c_uint64, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                                               C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -436,6 +439,21 @@ class GpuClosure:
     def last_closure_stats(self):
         a, b = C.c_int64(), C.c_int64()
         lib().uqk_last_closure_stats(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def set_factor_reuse(self, on=True):
+        """Mixed closure: keep each cell's inverse Hessian and take refined
+        steps on it until the contraction weakens (on by default for MIXED)."""
+        self._check(lib().uqk_set_factor_reuse(self.h, int(bool(on))), "uqk_set_factor_reuse")
+
+    def invalidate_factors(self):
+        """Forget the kept inverses (the next closure factors every cell)."""
+        self._check(lib().uqk_invalidate_factors(self.h), "uqk_invalidate_factors")
+
+    def last_closure_factorizations(self):
+        """(Cholesky factorisations, steps on a kept inverse) of the last call or run."""
+        a, b = C.c_int64(), C.c_int64()
+        lib().uqk_last_closure_factorizations(self.h, C.byref(a), C.byref(b))
         return a.value, b.value
 
     def set_kernel_timing(self, on=True):

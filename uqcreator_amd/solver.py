@@ -44,6 +44,7 @@ class GpuMomentSolver:
     def set_state(self, u, lam):
         self.u.upload(u)
         self.lam.upload(lam)
+        self.cl.invalidate_factors()  # the kept inverses belong to the old duals
 
     def step(self):
         nfail, dt, res = self.cl.step(self.prob, self.dmesh, self.u, self.lam, self.uq, self.u_scratch, self.status,
