@@ -240,8 +240,8 @@ int32_t uqk_device_count(void);
 /* Milliseconds spent in the closure kernel by the last uqk_closure_solve /
  * uqk_step call, timed with HIP events on the launch stream. */
 double uqk_last_closure_ms(const uqk_ctx* ctx);
-/* Totals of the last closure call: Newton steps (Hessian factorisations) and
- * gradient evaluations (line-search trials included) over all cells; after
+/* Totals of the last closure call: Newton steps (those on a reused
+ * factorisation included) and gradient evaluations (line-search trials included) over all cells; after
  * uqk_run_steps, the totals over the run's steps. */
 int uqk_last_closure_stats(const uqk_ctx* ctx, int64_t* newton_steps, int64_t* gradient_evaluations);
 /* Reuse of the factorisation in the mixed closure (no counterpart in the
@@ -254,9 +254,12 @@ int uqk_last_closure_stats(const uqk_ctx* ctx, int64_t* newton_steps, int64_t* g
  * FP64 accept test); otherwise, and after a contraction clearly worse than a
  * damped Newton step's (above 1.2 (1 - alpha) + 1e-3), the cell's current
  * Hessian is factored and the reference Newton step runs.  Memory per cell:
- * n (n + 1) floats and n (n + 3) / 2 doubles.  Single closure calls run the
- * reference Newton.  Results match the plain mixed closure to the Newton
- * tolerance, not bit for bit.  uqk_invalidate_factors forgets the kept
+ * n (n + 1) floats and n (n + 3) / 2 doubles.  In a single closure call
+ * (wave kernel, uniform orders, damped Newton: alpha < 1, whose steps
+ * contract linearly anyway) a cell's Newton steps after a factorisation
+ * solve with that factor (d = (U^T U)^{-1} g) under the same accept and
+ * contraction tests, with no memory beyond the call.  Results match the
+ * plain mixed closure to the Newton tolerance, not bit for bit.  uqk_invalidate_factors forgets the kept
  * inverses (after replacing the duals wholesale). */
 int uqk_set_factor_reuse(uqk_ctx* ctx, int32_t on);
 int uqk_invalidate_factors(uqk_ctx* ctx);

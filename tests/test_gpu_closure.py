@@ -234,6 +234,8 @@ def test_mixed_precision_closure(oracle, kind, d, deg, q):
     st_o, it_o = oracle.solve_closure(c, b.phi, b.phiw, u, lam_o)
     scale = np.abs(lam_star).max(axis=1, keepdims=True)
     cl = native.GpuClosure(kind, b, precision=native.MIXED, **gkw)
+    # plain mixed Newton: a factorisation at every step
+    cl.set_factor_reuse(False)
     nfail, lam, st, it = _run_gpu(cl, u, lam0)
     assert cl.last_kernel() == native.KERNEL_WAVE
     assert nfail == 0 and (st == 0).all() and (st_o == 0).all()
@@ -241,9 +243,25 @@ def test_mixed_precision_closure(oracle, kind, d, deg, q):
     assert (np.abs(lam - lam_star) / scale).max() < 1e-7
     # the FP32 Hessian changes the Newton path only slightly
     assert it.mean() <= it_o.mean() + 1.0
-    # a single closure call factors at every Newton step
     factors, lagged = cl.last_closure_factorizations()
     assert lagged == 0 and factors == it.sum()
+    # with the reuse of the factorisation within the call (damped Newton
+    # only: Euler1D, alpha 0.5): the same fixed point to the Newton
+    # tolerance; steps on a kept factor are accepted only when ||grad L||
+    # falls, so every cell still ends converged
+    cl.set_factor_reuse(True)
+    nfail, lam_r, st_r, it_r = _run_gpu(cl, u, lam0)
+    factors_r, lagged_r = cl.last_closure_factorizations()
+    print("reuse: factorisations", factors_r, "lagged", lagged_r, "iterations", it_r.sum(), "plain", it.sum(),
+          flush=True)
+    assert nfail == 0 and (st_r == 0).all()
+    assert (np.abs(lam_r - lam_o) / scale).max() < 1e-8
+    assert (np.abs(lam_r - lam_star) / scale).max() < 1e-7
+    assert factors_r + lagged_r <= it_r.sum()
+    if kind == native.EULER1D:  # most steps run on a kept factor
+        assert lagged_r > it_r.sum() // 2 and factors_r < it.sum() // 2
+    else:  # undamped Newton: the reference step throughout
+        assert lagged_r == 0 and (it_r == it).all()
 
 
 def test_autotuned_launch_shape_gives_identical_duals(tmp_path):

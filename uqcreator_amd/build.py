@@ -42,6 +42,7 @@ EXTRA_FLAGS = {
     # for the mixed closure, the same for FP64); closure_wave_reuse.hip keeps
     # the default one, which spills less there
     "closure_wave.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
+    "closure_wave_lag.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
 }
 
 

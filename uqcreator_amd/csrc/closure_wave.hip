@@ -2,14 +2,18 @@
 // Newton closure (closure_wave_kernel.h): FP64 (PREC 0) and mixed (PREC 1);
 // the mixed closure with kept inverses (PREC 2) and the kernel that turns a
 // time step's new Cholesky factors into those inverses live in
-// closure_wave_reuse.hip.
+// closure_wave_reuse.hip, the mixed closure reusing factors within one call
+// (PREC 4) in closure_wave_lag.hip.
 #include "closure_wave_kernel.h"
 
 namespace uqk {
 
-// PREC 2 launches (closure_wave_reuse.hip); false when the shape has none
+// PREC 2 launches (closure_wave_reuse.hip) and PREC 4 launches
+// (closure_wave_lag.hip); false when the shape has none
 bool newton_wave_reuse_launch(int kind, int M, int Q, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
                               hipError_t* err, const LaunchShape& shape);
+bool newton_wave_lag_launch(int kind, int M, int Q, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
+                            hipError_t* err, const LaunchShape& shape);
 
 template <int KIND, int M, int Q>
 static bool wave_try(int kind, int m, int q, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
@@ -21,6 +25,8 @@ static bool wave_try(int kind, int m, int q, const NewtonArgs& a, int max_lds, i
     }
     // the time loop's mixed closure with kept inverses
     if (a.pinv_mode == 1 && newton_wave_reuse_launch(kind, m, q, a, max_lds, num_cu, st, err, shape)) return true;
+    // one closure call's mixed closure reusing each cell's factor within the call
+    if (a.pinv_mode == 2 && newton_wave_lag_launch(kind, m, q, a, max_lds, num_cu, st, err, shape)) return true;
     wave_go<KIND, M, Q, 1>(a, max_lds, num_cu, st, err, shape);
     return true;
 }

@@ -78,6 +78,9 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 // accept test, Cholesky and solve stay FP64.  PREC 2: PREC 1 with the reuse
 // of the factorisation across time steps (the time loop's mixed closure; its
 // own instantiation, so that the single-call closure keeps its registers).
+// PREC 4: PREC 1 with the reuse of the factorisation within one closure call
+// (a cell's later Newton steps solve with its last factor while they
+// contract).
 template <int KIND, int M, int Q, int PREC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) newton_wave_kernel(NewtonArgs a) {
     using W = WaveShape<KIND, M, Q>;
@@ -509,6 +512,30 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
             return true;
         };
 
+        // d <- (U^T U)^{-1} v with the factor of the launch's last Cholesky
+        // still in Hs and rdiag (PREC 4): U^T y = v, then U x = y, lane r
+        // owning y_r / x_r; each step broadcasts x_i through SGPRs and reads
+        // row i of U (forward) or column i (back) from LDS, off the chain
+        auto factor_resolve = [&](double v) -> double {
+            // phase: matvec
+            double yr = lane < N ? v : 0.0;
+            const double rd = lane < N ? rdiag[lane] : 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const double ui = (lane > i && lane < N) ? Hs[lane * (lane + 1) / 2 + i] : 0.0;
+                const double xi = readlane_f64(yr * rd, i);
+                yr = lane == i ? xi : fma(-ui, xi, yr);
+            }
+#pragma unroll
+            for (int i = N - 1; i >= 0; --i) {
+                const double ui = lane < i ? Hs[i * (i + 1) / 2 + lane] : 0.0;
+                const double xi = readlane_f64(yr * rd, i);
+                yr = lane == i ? xi : fma(-ui, xi, yr);
+            }
+            // phase end: matvec
+            return yr;
+        };
+
         // ---- reuse of the factorisation (mixed precision) ----------------
         // The kept approximation of the inverse Hessian is P = H^{-1} of an
         // earlier step's factorisation (the FP32-assembled Hessian, factored
@@ -639,6 +666,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                 refactor = a.pinv_ok[cell] != 1;
             }
         }
+        // PREC 4 (one closure call): the factor of this cell's last Cholesky
+        // stays in Hs, and the next steps solve with it (lagged steps) until
+        // one contracts poorly or is rejected
+        if constexpr (PREC == 4) reuse = c.max_iter != 1;
         for (int it = 0; status < 0 && it < c.max_iter; ++it) {
             if (it > 0) g = gnew;
             ++iters;
@@ -649,8 +680,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
             // factors its own Hessian and runs the reference Newton step
             bool lagged = false;
             double Lk0[S];
-            if constexpr (PREC == 2) lagged = reuse && !refactor;
+            if constexpr (PREC == 2 || PREC == 4) lagged = reuse && !refactor;
             if (lagged) {
+                if constexpr (PREC == 4) {
+                    d = factor_resolve(g);
+#pragma unroll
+                    for (int s = 0; s < S; ++s) Lk0[s] = Lk[s];
+                }
                 if constexpr (PREC == 2) {
                     if (!vlds) {
                         load_inverse(a.pinv + cell * (long)(N * (N + 1)));
@@ -670,6 +706,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                     break;
                 }
                 ++w_factor;
+                if constexpr (PREC == 4) refactor = false;
                 if constexpr (PREC == 2) {
                     if (reuse) {
                         store_factor(a.ufac + cell * (long)(W::HP + N));
@@ -751,7 +788,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
         if (status < 0) status = UQK_STATUS_MAX_ITERATIONS;
         // the cell's slot for the next launch: a new factor to invert (2),
         // the kept inverse still fresh (1), or none (0)
-        if (reuse && lane == 0) {
+        if (PREC == 2 && reuse && lane == 0) {
             a.pinv_ok[cell] = factored ? 2 : (refactor ? 0 : 1);
             if (factored) a.plist[atomicAdd(a.plist_count, 1ull)] = (int)cell;
         }

@@ -190,6 +190,12 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         a.plist = ctx->pinv_list;
         a.plist_count = counters + 7;
         a.pinv_slots = ctx->pinv_cell_cap;
+    } else if (ctx->precision == UQK_MIXED && ctx->factor_reuse && !nt && ctx->c.max_iter != 1 && !factors_per_cell &&
+               ctx->c.alpha < 1.0) {
+        // (wave kernel) reuse within the call, for the damped Newton only: its
+        // steps contract linearly, by about 1 - alpha, with a kept factor as
+        // with a fresh one; the undamped Newton would lose its quadratic rate
+        a.pinv_mode = 2;
     }
     const bool want_dt = dt_out && dtp && dx_param && uq_out && u_out && ctx->c.max_iter != 1;
     if (want_dt) {

@@ -43,15 +43,17 @@ struct NewtonArgs {
     // wins); *took += cells that started from lam_alt
     const double* lam_alt;
     unsigned long long* took;
-    // mixed precision, wave kernel, time loop: reuse of the factorisation
-    // across steps (pinv_mode 1).  A cell whose pinv_ok is 1 holds a kept
-    // inverse Hessian in pinv (FP32, n x n, row stride n + 1) and takes
-    // Newton steps d = P g refined once with the FP64 Hessian-vector
+    // mixed precision, wave kernel: reuse of the factorisation.
+    // pinv_mode 1 (time loop, across steps): a cell whose pinv_ok is 1 holds
+    // a kept inverse Hessian in pinv (FP32, n x n, row stride n + 1) and
+    // takes Newton steps d = P g refined once with the FP64 Hessian-vector
     // product; a factored step stores its factor in ufac (packed upper U,
     // then 1/U_rr, FP64), sets pinv_ok to 2 and appends the cell to
     // plist[(*plist_count)++], and factor_inverse_kernel turns the listed
-    // factors into the next inverses.  nfactor[0] += Cholesky
-    // factorisations, nfactor[1] += steps on a kept inverse.
+    // factors into the next inverses.  pinv_mode 2 (one closure call): a
+    // cell's later Newton steps solve with the factor of its last Cholesky
+    // while they contract.  nfactor[0] += Cholesky factorisations,
+    // nfactor[1] += accepted steps on a kept factor or inverse.
     float* pinv;
     double* ufac;
     int* pinv_ok;
