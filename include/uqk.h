@@ -332,8 +332,11 @@ int uqk_mlp_predict(uqk_mlp* net, int64_t n, const double* u, double* lambda_out
  * ||grad L||_2 (non-finite never wins).  Returns the number of cells that
  * took lambda_alt. */
 int64_t uqk_warm_start_select(uqk_ctx* ctx, int64_t n, const double* u, double* lambda, const double* lambda_alt);
-/* Attach (or detach with NULL) a network: every later closure of ctx with
- * uniform orders first predicts a start and keeps the better one per cell. */
+/* Attach (or detach with NULL) a network: every later closure of ctx first
+ * predicts a start and keeps the better one per cell.  Under per-cell orders
+ * the moments past a cell's order enter the network as zero, the predicted
+ * duals there are zero, and the choice compares ||grad L|| over the cell's
+ * own moments; training pairs from such runs are zero-padded the same way. */
 int uqk_set_warm_start(uqk_ctx* ctx, uqk_mlp* net);
 
 /* ---- whole runs from an input file ------------------------------------- */

@@ -293,6 +293,7 @@ struct FeatArgs {
     const long* idx;          // row -> cell (null: identity)
     const double* u;          // cells x M x S
     const double* lam;        // cells x M x S (null: no targets)
+    const int* nt;            // per-cell orders: moments per cell (null: M)
     int M;
     uqk::ClosureParams cp;
     int K0, NO;
@@ -372,7 +373,8 @@ __global__ __launch_bounds__(256) void features_kernel(FeatArgs a) {
         float x = 0.f;
         if (rs.ok[row] && f < nf) {
             const long c = a.idx ? a.idx[r] : r;
-            x = (float)(a.u[(size_t)c * nf + f] / rs.D[row][f % S]);
+            // moments past the cell's order are absent: a zero moment
+            if (!a.nt || f / S < a.nt[c]) x = (float)(a.u[(size_t)c * nf + f] / rs.D[row][f % S]);
             if (a.mu_x) x = (x - a.mu_x[f]) / a.sd_x[f];
         }
         if (a.xh) a.xh[(size_t)r * a.K0 + f] = (__bf16)x;
@@ -405,10 +407,11 @@ __global__ __launch_bounds__(256) void features_kernel(FeatArgs a) {
 }
 
 // lambda = lambda_lin + (mu_t + sd_t y) rho0/D_s; NaN where the mean state is
-// not physical (the selection then keeps the plain start).  A block covers
-// kRowsPerBlock cells; threads over (cell, moment) pairs.
+// not physical (the selection then keeps the plain start); zero past a
+// cell's order nt[c] (per-cell orders).  A block covers kRowsPerBlock cells;
+// threads over (cell, moment) pairs.
 template <int KIND>
-__global__ __launch_bounds__(256) void apply_kernel(long n, const double* u, const float* y, int M,
+__global__ __launch_bounds__(256) void apply_kernel(long n, const double* u, const int* nt, const float* y, int M,
                                                     uqk::ClosureParams cp, int NO, const float* mu_t,
                                                     const float* sd_t, double* lam) {
     constexpr int S = uqk::KindInfo<KIND>::S;
@@ -425,7 +428,9 @@ __global__ __launch_bounds__(256) void apply_kernel(long n, const double* u, con
         for (int s = 0; s < S; ++s) {
             const int f = i * S + s;
             double v = NAN;
-            if (rs.ok[row]) {
+            if (nt && i >= nt[c]) {
+                v = 0.0;  // duals past the cell's order stay zero
+            } else if (rs.ok[row]) {
                 const double tv = (double)mu_t[f] + (double)sd_t[f] * (double)y[(size_t)c * NO + f];
                 v = z[s] + tv * rs.D[row][0] / rs.D[row][s];
             }
@@ -437,14 +442,15 @@ __global__ __launch_bounds__(256) void apply_kernel(long n, const double* u, con
 // Selection: per cell (one wave), ||grad L||_2 at lambda and at lambda_alt
 // (closure.cpp:234-253 gradient, (:195-203) norm); the smaller one stays in
 // lambda.  Non-finite norms never win.  counter += cells that took lambda_alt.
+// Per-cell orders (nt not null): the sums run over the cell's nt[c] moments.
 // Persistent workgroups: phi and phiW are staged in LDS once per workgroup,
 // each wave walks cells with a grid stride and stages the candidate duals of
 // its cell in LDS (the sums run in the same order as a plain loop over k, i).
 template <int KIND>
 __global__ __launch_bounds__(256) void select_kernel(uqk::ClosureParams cp, long ncell, int M, int Q,
                                                      const double* phi, const double* phiw, const double* u,
-                                                     double* lam, const double* lam_alt, unsigned long long* counter,
-                                                     const int* active) {
+                                                     double* lam, const double* lam_alt, const int* nt,
+                                                     unsigned long long* counter, const int* active) {
     constexpr int S = uqk::KindInfo<KIND>::S;
     if (active && *active == 0) return;
     extern __shared__ double sel_lds[];
@@ -462,6 +468,7 @@ __global__ __launch_bounds__(256) void select_kernel(uqk::ClosureParams cp, long
     unsigned long long took = 0;
     for (long c = (long)blockIdx.x * 4 + w; c < ncell; c += (long)gridDim.x * 4) {
         const double* uc = u + (size_t)c * nf;
+        const int ntc = nt ? nt[c] : M;
         double norm[2];
         for (int cand = 0; cand < 2; ++cand) {
             const double* l = (cand ? lam_alt : lam) + (size_t)c * nf;
@@ -470,7 +477,7 @@ __global__ __launch_bounds__(256) void select_kernel(uqk::ClosureParams cp, long
             for (int k = lane; k < Q; k += 64) {
                 double L[S], U[S];
                 for (int s = 0; s < S; ++s) L[s] = 0.0;
-                for (int i = 0; i < M; ++i) {
+                for (int i = 0; i < ntc; ++i) {
                     const double p = tphi[k * M + i];
                     for (int s = 0; s < S; ++s) L[s] += lw[i * S + s] * p;
                 }
@@ -479,7 +486,7 @@ __global__ __launch_bounds__(256) void select_kernel(uqk::ClosureParams cp, long
             }
             uqk::wave_sync();
             double acc = 0.0;
-            for (int f = lane; f < nf; f += 64) {
+            for (int f = lane; f < ntc * S; f += 64) {
                 const int i = f / S, s = f % S;
                 double gsum = 0.0;
                 for (int k = 0; k < Q; ++k) gsum += uq[k * S + s] * tphiw[k * M + i];
@@ -703,12 +710,13 @@ int backward(uqk_mlp* n, long rows, int n_real) {
 }
 
 int launch_features(uqk_mlp* n, long rows, const long* idx, const double* u, const double* lam, bool standardise,
-                    __bf16* xh, float* xf, float* tf) {
+                    __bf16* xh, float* xf, float* tf, const int* nt = nullptr) {
     FeatArgs a{};
     a.n = rows;
     a.idx = idx;
     a.u = u;
     a.lam = lam;
+    a.nt = nt;
     a.M = n->M;
     a.cp = n->ctx->c;
     a.K0 = n->K0;
@@ -737,7 +745,7 @@ int launch_features(uqk_mlp* n, long rows, const long* idx, const double* u, con
 namespace uqk {
 
 // features -> network -> duals for ncell cells, in chunks of 64K rows; enqueue only
-int enqueue_predict(uqk_mlp* n, long ncell, const double* u, double* lambda_out) {
+int enqueue_predict(uqk_mlp* n, long ncell, const double* u, double* lambda_out, const int* nt) {
     if (ncell == 0) return 0;
     const long chunk = kPredictChunk;
     if (int rc = ensure_rows(n, std::min<long>(ncell, chunk))) return rc;
@@ -747,15 +755,16 @@ int enqueue_predict(uqk_mlp* n, long ncell, const double* u, double* lambda_out)
     for (long c0 = 0; c0 < ncell; c0 += chunk) {
         const long rows = std::min<long>(chunk, ncell - c0);
         const double* uc = u + (size_t)c0 * n->M * n->S;
-        if (int rc = launch_features(n, rows, nullptr, uc, nullptr, true, n->act[0], nullptr, nullptr)) return rc;
+        const int* ntc = nt ? nt + c0 : nullptr;
+        if (int rc = launch_features(n, rows, nullptr, uc, nullptr, true, n->act[0], nullptr, nullptr, ntc)) return rc;
         if (int rc = forward(n, rows, false)) return rc;
         double* lc = lambda_out + (size_t)c0 * n->M * n->S;
         if (n->ctx->kind == UQK_EULER1D)
-            hipLaunchKernelGGL(apply_kernel<UQK_EULER1D>, dim3(blocks(rows, kRowsPerBlock)), dim3(256), 0, st, rows, uc, n->y,
-                               n->M, n->ctx->c, n->NO, mu_t, sd_t, lc);
+            hipLaunchKernelGGL(apply_kernel<UQK_EULER1D>, dim3(blocks(rows, kRowsPerBlock)), dim3(256), 0, st, rows, uc,
+                               ntc, n->y, n->M, n->ctx->c, n->NO, mu_t, sd_t, lc);
         else
-            hipLaunchKernelGGL(apply_kernel<UQK_EULER2D>, dim3(blocks(rows, kRowsPerBlock)), dim3(256), 0, st, rows, uc, n->y,
-                               n->M, n->ctx->c, n->NO, mu_t, sd_t, lc);
+            hipLaunchKernelGGL(apply_kernel<UQK_EULER2D>, dim3(blocks(rows, kRowsPerBlock)), dim3(256), 0, st, rows, uc,
+                               ntc, n->y, n->M, n->ctx->c, n->NO, mu_t, sd_t, lc);
     }
     MLP_HIP(n, hipGetLastError());
     return 0;
@@ -764,7 +773,7 @@ int enqueue_predict(uqk_mlp* n, long ncell, const double* u, double* lambda_out)
 // Selection kernel only (no readback): capturable.  counters[4] = cells that took
 // lambda_alt.
 int enqueue_warm_select(uqk_ctx* ctx, long ncell, const double* u, double* lambda, const double* lambda_alt,
-                        const int* active) {
+                        const int* active, const int* nt) {
     if (ctx->kind != UQK_EULER1D && ctx->kind != UQK_EULER2D)
         return fail_ctx(ctx, -1, "warm start: supported for the Euler closures only");
     unsigned long long* cnt = ctx->d_counters + 4;
@@ -776,10 +785,10 @@ int enqueue_warm_select(uqk_ctx* ctx, long ncell, const double* u, double* lambd
     const dim3 grid((unsigned)(want < cap ? (want > 0 ? want : 1) : cap));
     if (ctx->kind == UQK_EULER1D)
         hipLaunchKernelGGL(select_kernel<UQK_EULER1D>, grid, dim3(256), lds, ctx->stream, ctx->c, ncell, ctx->M,
-                           ctx->Q, ctx->d_phi, ctx->d_phiw, u, lambda, lambda_alt, cnt, active);
+                           ctx->Q, ctx->d_phi, ctx->d_phiw, u, lambda, lambda_alt, nt, cnt, active);
     else
         hipLaunchKernelGGL(select_kernel<UQK_EULER2D>, grid, dim3(256), lds, ctx->stream, ctx->c, ncell, ctx->M,
-                           ctx->Q, ctx->d_phi, ctx->d_phiw, u, lambda, lambda_alt, cnt, active);
+                           ctx->Q, ctx->d_phi, ctx->d_phiw, u, lambda, lambda_alt, nt, cnt, active);
     UQK_HIP_CTX(ctx, hipGetLastError());
     return 0;
 }
@@ -799,12 +808,12 @@ int prepare_warm_start(uqk_ctx* ctx, long ncell) {
     return ensure_rows(n, std::min<long>(ncell, kPredictChunk));
 }
 
-int enqueue_warm_predict(uqk_ctx* ctx, long ncell, const double* u, const double** lam_alt) {
+int enqueue_warm_predict(uqk_ctx* ctx, long ncell, const double* u, const double** lam_alt, const int* nt) {
     uqk_mlp* n = ctx->warm;
     *lam_alt = nullptr;
     if (!n) return 0;
     if (int rc = prepare_warm_start(ctx, ncell)) return rc;
-    if (int rc = enqueue_predict(n, ncell, u, n->lam_alt)) return rc;
+    if (int rc = enqueue_predict(n, ncell, u, n->lam_alt, nt)) return rc;
     *lam_alt = n->lam_alt;
     return 0;
 }
@@ -1054,7 +1063,7 @@ int uqk_mlp_train(uqk_mlp* n, int64_t ncell, const double* u, const double* lamb
 
 int uqk_mlp_predict(uqk_mlp* n, int64_t ncell, const double* u, double* lambda_out) {
     if (!n || ncell < 0) return -1;
-    if (int rc = uqk::enqueue_predict(n, ncell, u, lambda_out)) return rc;
+    if (int rc = uqk::enqueue_predict(n, ncell, u, lambda_out, nullptr)) return rc;
     MLP_HIP(n, hipStreamSynchronize(n->ctx->stream));
     return 0;
 }
@@ -1062,7 +1071,7 @@ int uqk_mlp_predict(uqk_mlp* n, int64_t ncell, const double* u, double* lambda_o
 int64_t uqk_warm_start_select(uqk_ctx* ctx, int64_t ncell, const double* u, double* lambda, const double* lambda_alt) {
     if (!ctx || ncell < 0) return -1;
     if (ncell == 0) return 0;
-    if (int rc = uqk::enqueue_warm_select(ctx, ncell, u, lambda, lambda_alt, nullptr)) return rc;
+    if (int rc = uqk::enqueue_warm_select(ctx, ncell, u, lambda, lambda_alt, nullptr, nullptr)) return rc;
     unsigned long long h = 0;
     UQK_HIP_CTX(ctx, hipMemcpyAsync(&h, ctx->d_counters + 4, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
     UQK_HIP_CTX(ctx, hipStreamSynchronize(ctx->stream));

@@ -151,12 +151,12 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     // [0] queue head, [1] failed cells, [2] Newton steps, [3] gradients, [4] cells that took the learned start
     if (!prezeroed) UQK_HIP(ctx, uqk::launch_fill_u64(counters, uqk::kClosureCounters, ctx->stream));
     if (n <= 0) return 0;
-    // learned warm start (uniform orders): the network's prediction; the wave
-    // kernel selects the start per cell itself, the other kernels after a
-    // selection pass
+    // learned warm start: the network's prediction (per-cell orders: masked
+    // to each cell's moments); the wave kernel selects the start per cell
+    // itself, the other kernels after a selection pass
     const double* lam_alt = nullptr;
-    if (ctx->warm && !nt)
-        if (int rc = uqk::enqueue_warm_predict(ctx, n, u, &lam_alt)) return rc;
+    if (ctx->warm)
+        if (int rc = uqk::enqueue_warm_predict(ctx, n, u, &lam_alt, nt)) return rc;
     uqk::NewtonArgs a{};
     a.c = ctx->c;
     a.M = ctx->M;
@@ -239,7 +239,7 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         a.lam_alt = nullptr;
     }
     if (lam_alt)
-        if (int rc = uqk::enqueue_warm_select(ctx, n, u, lam, lam_alt, active)) return rc;
+        if (int rc = uqk::enqueue_warm_select(ctx, n, u, lam, lam_alt, active, nt)) return rc;
     const bool block_ok = uqk::newton_block_fits(ctx->S, ctx->M, ctx->Q, (int)ctx->max_lds);
     if (ctx->kernel == UQK_KERNEL_BLOCK || (ctx->kernel == UQK_KERNEL_AUTO && ctx->S * ctx->M >= 64 && block_ok)) {
         if (!block_ok) return fail(ctx, -3, "closure too large for the workgroup kernel");

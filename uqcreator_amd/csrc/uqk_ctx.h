@@ -87,8 +87,12 @@ inline void invalidate_graph(uqk_ctx* ctx) {
 // (outside graph capture, which cannot allocate).
 int prepare_warm_start(uqk_ctx* ctx, long ncell);
 // learned warm start (mlp.hip): the attached network's starting duals for the
-// moments u into its own buffer (*lam_alt; null without a network); enqueue only.
-int enqueue_warm_predict(uqk_ctx* ctx, long ncell, const double* u, const double** lam_alt);
+// moments u into its own buffer (*lam_alt; null without a network); enqueue
+// only.  nt: per-cell orders (moments past a cell's order read as zero, its
+// duals there are zero), or null.
+int enqueue_warm_predict(uqk_ctx* ctx, long ncell, const double* u, const double** lam_alt, const int* nt);
+// per cell, the start with the smaller ||grad L|| (over the cell's nt[c]
+// moments when nt is given) into lambda
 int enqueue_warm_select(uqk_ctx* ctx, long ncell, const double* u, double* lambda, const double* lambda_alt,
-                        const int* active);
+                        const int* active, const int* nt);
 }  // namespace uqk
