@@ -293,10 +293,12 @@ int uqk_load_tuning(uqk_ctx* ctx, const char* json_path);
 
 /* ---- learned warm start (no counterpart in the reference) --------------- */
 /* A network that maps a cell's moments to a starting dual for the Newton
- * closure of an Euler context (UQK_EULER1D / UQK_EULER2D).  Inputs are the
- * moments scaled by the cell's mean state (rho0, rho0 sqrt(T0), rho0 T0),
- * outputs the correction to the plain start (DS(u_0), 0, ...) in the same
- * scaling; four hidden layers of `hidden` units with GELU, BF16 weights on
+ * closure of an Euler or shallow-water context (UQK_EULER1D, UQK_EULER2D,
+ * UQK_SHALLOW_WATER2D, UQK_SHALLOW_WATER2D_BARRIER).  Inputs are the moments
+ * scaled by the cell's mean state (Euler: rho0, rho0 sqrt(T0), rho0 T0;
+ * shallow water: h0, h0 sqrt(g h0)), outputs the correction to the
+ * linearised start (the plain start (DS(u_0), 0, ...) and one Newton step)
+ * made dimensionless by the same mean state; four hidden layers of `hidden` units with GELU, BF16 weights on
  * MFMA with FP32 accumulation, trained on the GPU with Adam.  All array
  * arguments are device pointers unless named host_. */
 typedef struct uqk_mlp uqk_mlp;

@@ -1,9 +1,10 @@
-"""Learned warm start of the 2D Euler closure on the GPU: the feature map and
-the linearised start against their definitions (oracle DS / DU), the gain of
-a trained network on held-out cells with uniform orders (wave kernel, the
-selection fused into it), and the same under per-cell orders (nested total
-degrees, momentsolver.cpp:88,104-117), where the moments past a cell's order
-enter the network as zero and its duals there stay zero."""
+"""Learned warm start of the 2D Euler and shallow-water closures on the GPU:
+the feature map and the linearised start against their definitions (oracle
+DS / DU), the gain of a trained network on held-out cells with uniform orders
+(wave kernel, the selection fused into it; workgroup kernel for the
+dam-break closure, n = 105), and the same under per-cell orders (nested
+total degrees, momentsolver.cpp:88,104-117), where the moments past a cell's
+order enter the network as zero and its duals there stay zero."""
 import numpy as np
 import pytest
 
@@ -158,3 +159,83 @@ def test_warm_start_under_per_cell_orders(oracle):
     for j in range(n_test):
         assert not warm[j, nt_test[j]:].any()
     assert warm_steps < plain_steps
+
+
+def _sw_cells(oracle, b, n, seed, barrier, spread=0.02):
+    rng = np.random.default_rng(seed)
+    okind = oracle.SW2D_BARRIER if barrier else oracle.SW2D
+    c = oracle.closure(okind, eps=1e-8, safe=1, barrier=barrier)
+    lam_star = np.zeros((n, b.M, 3))
+    for j in range(n):
+        h, vx, vy = rng.uniform(1, 3), rng.uniform(-1, 1), rng.uniform(-1, 1)
+        lam_star[j, 0] = oracle.DS(c, np.array([h, h * vx, h * vy]))
+        lam_star[j, 1:] = rng.normal(0, spread, (b.M - 1, 3)) * np.abs(lam_star[j, 0])
+    uq = oracle.eval_uq(c, b.phi, lam_star)
+    u = oracle.reconstruct(3, b.phiw, np.ascontiguousarray(uq))
+    lam0 = np.zeros_like(lam_star)
+    for j in range(n):
+        lam0[j, 0] = oracle.DS(c, u[j, 0])
+    return c, u, lam_star, lam0
+
+
+SW_CASES = [(native.SHALLOW_WATER2D, 0.0, (2, [3], 5)), (native.SHALLOW_WATER2D_BARRIER, 0.01, (3, [4], 5))]
+
+
+@pytest.mark.parametrize("kind,barrier,basis", SW_CASES)
+def test_shallow_water_features_and_linear_start(oracle, kind, barrier, basis):
+    """Scales (h0, h0 a0, h0 a0), a0 = sqrt(g h0); targets times (1/a0^2,
+    1/a0, 1/a0); a zero network predicts the linearised start DS(u_0),
+    DU(DS(u_0))^{-1} u_i from the oracle's DS and DU."""
+    b = native.Basis(*basis)
+    c, u, lam_star, lam0 = _sw_cells(oracle, b, 256, seed=41, barrier=barrier)
+    cl = native.GpuClosure(kind, b, barrier=barrier, epsilon=1e-8, safe_line_search=1)
+    net = native.WarmStartNet(cl, hidden=64, seed=3)
+    nf = 3 * b.M
+    du, dl = native.DeviceArray.from_host(u), native.DeviceArray.from_host(lam_star)
+    x, t = net.features(du, dl, standardise=False)
+    x, t = x.download(), t.download()
+    print("case: shallow-water features kind", kind, "M", b.M, flush=True)
+    for j in (0, 77, 255):
+        h = u[j, 0, 0]
+        a = np.sqrt(9.81 * h)
+        D = np.array([h, h * a, h * a])
+        ts = np.array([1 / a ** 2, 1 / a, 1 / a])
+        np.testing.assert_allclose(x[j, :nf], (u[j] / D).ravel(), rtol=2e-7, atol=1e-7)
+        assert not x[j, nf:].any()
+        lin = _linear_start(oracle, c, u[j])
+        np.testing.assert_allclose(t[j, :nf], ((lam_star[j] - lin) * ts).ravel(), rtol=2e-5, atol=2e-6)
+    p, nrm = net.get_params()
+    net.set_params(np.zeros_like(p), nrm)
+    lp = net.predict(du).download()
+    for j in (0, 77, 255):
+        np.testing.assert_allclose(lp[j], _linear_start(oracle, c, u[j]), rtol=1e-11, atol=1e-13)
+
+
+def test_shallow_water_barrier_warm_start_gain(oracle):
+    """The dam-break closure (barrier 0.01, n = 105, 5^3 points, workgroup
+    kernel): a trained network keeps the fixed point and saves Newton steps
+    on held-out cells."""
+    b = native.Basis(3, [4], 5)
+    _, u, lam_star, lam0 = _sw_cells(oracle, b, 1 << 13, seed=51, barrier=0.01)
+    cl = native.GpuClosure(native.SHALLOW_WATER2D_BARRIER, b, barrier=0.01, epsilon=1e-8, safe_line_search=1)
+    du = native.DeviceArray.from_host(u)
+    dl = native.DeviceArray.from_host(lam0)
+    print("case: shallow-water barrier warm start train 8192 pairs", flush=True)
+    assert cl.solve(du, dl) == 0
+    net = native.WarmStartNet(cl, hidden=128, seed=8)
+    _train(net, du, dl)
+    _, ut, ls_t, l0_t = _sw_cells(oracle, b, 1024, seed=52, barrier=0.01)
+    plain = l0_t.copy()
+    st_p, it_p = cl.solve_host(ut, plain)
+    assert cl.last_kernel() == native.KERNEL_BLOCK
+    net.attach()
+    warm = l0_t.copy()
+    st_w, it_w = cl.solve_host(ut, warm)
+    net.detach()
+    assert (st_p == 0).all() and (st_w == 0).all()
+    scale = np.abs(ls_t).max(axis=1, keepdims=True)
+    err_plain = (np.abs(plain - ls_t) / scale).max()
+    err_warm = (np.abs(warm - ls_t) / scale).max()
+    print("Newton steps plain", int(it_p.sum()), "warm", int(it_w.sum()), "lambda err", err_plain, err_warm, flush=True)
+    assert err_plain < 1e-7 and err_warm < 1e-7
+    assert it_w.sum() < it_p.sum()

@@ -8,14 +8,15 @@
 // padded).  Weights are FP32 masters with BF16 copies; every product runs on
 // v_mfma_f32_16x16x32_bf16 with FP32 accumulation; Adam updates the masters.
 //
-// Features and targets (Euler closures, S = 3 or 4 states): from the cell
-// means rho0, m0, E0 take p0 = (gamma-1)(E0 - |m0|^2/(2 rho0)), T0 = p0/rho0
-// and the scales D = (rho0, rho0 sqrt(T0) [per momentum], rho0 T0).  The
-// input is u(i,s)/D_s; the target is (lambda - lambda_lin)(i,s) * D_s/rho0
-// with lambda_lin the linearised start (LinearStart below: the plain start
-// (DS(u_0), 0, ...) followed by one Newton step, in closed form).  Scaling u
-// by D maps the dual of the scaled problem affinely onto the original one,
-// so the inverse map lambda = lambda_lin + y * rho0/D_s holds at any scale.
+// Features and targets (Euler closures, S = 3 or 4 states; shallow water,
+// S = 3, with or without the barrier): from the cell mean u_0 take feature
+// scales D and target scales ts (cell_scales below; Euler: D = (rho0,
+// rho0 sqrt(T0) [per momentum], rho0 T0) with T0 = p0/rho0 and ts = D/rho0;
+// shallow water: D = (h0, h0 a0, h0 a0) with a0 = sqrt(g h0) and ts = (1/a0^2,
+// 1/a0, 1/a0)).  The input is u(i,s)/D_s; the target is (lambda -
+// lambda_lin)(i,s) * ts_s with lambda_lin the linearised start (LinearStart
+// below: the plain start (DS(u_0), 0, ...) followed by one Newton step, in
+// closed form), so lambda = lambda_lin + y / ts_s inverts it at any scale.
 // Both are standardised per feature with the training set's mean and std.
 #include <hip/hip_runtime.h>
 
@@ -24,6 +25,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "synth.h"
@@ -222,27 +224,69 @@ __global__ void to_bf16_kernel(const float* x, __bf16* y, long n) {
     if (i < n) y[i] = (__bf16)x[i];
 }
 
-// Euler cell scales and plain start: returns false for a non-physical mean.
-template <int S>
-__device__ __forceinline__ bool euler_scales(const double* u, double gm, double* D, double* ds) {
-    const double rho = u[0], E = u[S - 1];
-    double q2 = 0.0;
-    for (int s = 1; s < S - 1; ++s) q2 += u[s] * u[s];
-    const double p = (gm - 1.0) * (E - 0.5 * q2 / rho);
-    const double T = p / rho;
-    if (!(rho > 0.0) || !(T > 0.0) || !isfinite(T)) return false;
-    const double sq = sqrt(T);
-    D[0] = rho;
-    for (int s = 1; s < S - 1; ++s) D[s] = rho * sq;
-    D[S - 1] = rho * T;
-    // DS closed forms: EulerClosure::DS (eulerclosure.cpp:70-80) and
-    // EulerClosure2D::DS (eulerclosure2d.cpp:90-102), gamma negated there.
-    const double g = -gm;
-    const double e_int = E - q2 / (2.0 * rho);
-    ds[0] = (q2 + g * (2.0 * rho * E - q2)) / (-2.0 * rho * E + q2) - log(pow(rho, g) * e_int);
-    for (int s = 1; s < S - 1; ++s) ds[s] = -((2.0 * rho * u[s]) / (-2.0 * rho * E + q2));
-    ds[S - 1] = -(rho / e_int);
-    return true;
+// Cell scales and plain start from the cell's mean state u_0: feature
+// scales D (moments are divided by them), target scales ts (the dual
+// corrections are multiplied by them, which makes them dimensionless) and
+// ds = DS(u_0).  Returns false for a non-physical mean.
+//   Euler: D = (rho, rho sqrt(T), ..., rho T), ts = D / rho; DS closed forms
+//   of EulerClosure::DS (eulerclosure.cpp:70-80) and EulerClosure2D::DS
+//   (eulerclosure2d.cpp:90-102), gamma negated there.
+//   Shallow water: D = (h, h a, h a) with a = sqrt(g h), ts = (1/a^2, 1/a,
+//   1/a); DS inverts U (shallowwaterclosure2d.cpp:10-14): Lambda_1,2 = the
+//   velocity, Lambda_0 = g h - |v|^2 / 2 (with the barrier of the
+//   log-barrier entropy, g h - eps / h - |v|^2 / 2).
+template <int KIND>
+__device__ __forceinline__ bool cell_scales(const double* u, const uqk::ClosureParams& cp, double* D, double* ts,
+                                            double* ds) {
+    constexpr int S = uqk::KindInfo<KIND>::S;
+    if constexpr (KIND == UQK_EULER1D || KIND == UQK_EULER2D) {
+        const double gm = cp.gamma;
+        const double rho = u[0], E = u[S - 1];
+        double q2 = 0.0;
+        for (int s = 1; s < S - 1; ++s) q2 += u[s] * u[s];
+        const double p = (gm - 1.0) * (E - 0.5 * q2 / rho);
+        const double T = p / rho;
+        if (!(rho > 0.0) || !(T > 0.0) || !isfinite(T)) return false;
+        const double sq = sqrt(T);
+        D[0] = rho;
+        for (int s = 1; s < S - 1; ++s) D[s] = rho * sq;
+        D[S - 1] = rho * T;
+        for (int s = 0; s < S; ++s) ts[s] = D[s] / rho;
+        const double g = -gm;
+        const double e_int = E - q2 / (2.0 * rho);
+        ds[0] = (q2 + g * (2.0 * rho * E - q2)) / (-2.0 * rho * E + q2) - log(pow(rho, g) * e_int);
+        for (int s = 1; s < S - 1; ++s) ds[s] = -((2.0 * rho * u[s]) / (-2.0 * rho * E + q2));
+        ds[S - 1] = -(rho / e_int);
+        return true;
+    } else {
+        const double h = u[0];
+        const double a2 = cp.g * h;
+        if (!(h > 0.0) || !(a2 > 0.0) || !isfinite(a2)) return false;
+        const double a = sqrt(a2);
+        const double vx = u[1] / h, vy = u[2] / h;
+        D[0] = h;
+        D[1] = D[2] = h * a;
+        ts[0] = 1.0 / a2;
+        ts[1] = ts[2] = 1.0 / a;
+        double c0 = a2;
+        if constexpr (KIND == UQK_SHALLOW_WATER2D_BARRIER) c0 -= cp.barrier / h;
+        ds[0] = c0 - 0.5 * (vx * vx + vy * vy);
+        ds[1] = vx;
+        ds[2] = vy;
+        return isfinite(ds[0]) && isfinite(vx) && isfinite(vy);
+    }
+}
+
+// The closures the warm start supports, dispatched at run time.
+template <class F>
+bool with_warm_kind(int kind, F&& f) {
+    switch (kind) {
+    case UQK_EULER1D: f(std::integral_constant<int, UQK_EULER1D>()); return true;
+    case UQK_EULER2D: f(std::integral_constant<int, UQK_EULER2D>()); return true;
+    case UQK_SHALLOW_WATER2D: f(std::integral_constant<int, UQK_SHALLOW_WATER2D>()); return true;
+    case UQK_SHALLOW_WATER2D_BARRIER: f(std::integral_constant<int, UQK_SHALLOW_WATER2D_BARRIER>()); return true;
+    default: return false;
+    }
 }
 
 // Linearised start: at lambda = (DS(u_0), 0, ...) every quadrature point
@@ -313,6 +357,7 @@ constexpr int kRowsPerBlock = 64;
 template <int S>
 struct RowScales {
     double D[kRowsPerBlock][S];
+    double ts[kRowsPerBlock][S];
     double ds[kRowsPerBlock][S];
     double L[kRowsPerBlock][S][S];
     int ok[kRowsPerBlock];
@@ -328,11 +373,12 @@ __device__ void stage_rows(RowScales<uqk::KindInfo<KIND>::S>& rs, long n, long r
         int ok = 0;
         if (r < n) {
             const long c = idx ? idx[r] : r;
-            double D[S], ds[S];
+            double D[S], ts[S], ds[S];
             LinearStart<KIND> lin;
-            ok = euler_scales<S>(u + (size_t)c * M * S, cp.gamma, D, ds) && lin.factor(cp, ds);
+            ok = cell_scales<KIND>(u + (size_t)c * M * S, cp, D, ts, ds) && lin.factor(cp, ds);
             for (int s = 0; s < S; ++s) {
                 rs.D[t][s] = D[s];
+                rs.ts[t][s] = ts[s];
                 rs.ds[t][s] = ds[s];
                 for (int q = 0; q < S; ++q) rs.L[t][s][q] = lin.L[s][q];
             }
@@ -381,7 +427,7 @@ __global__ __launch_bounds__(256) void features_kernel(FeatArgs a) {
         if (a.xf) a.xf[(size_t)r * a.K0 + f] = x;
     }
     if (!a.lam) return;
-    // targets: the correction to the linearised start, scaled by D_s / rho_0
+    // targets: the correction to the linearised start, scaled by ts_s
     for (int t = threadIdx.x; t < kRowsPerBlock * a.M; t += blockDim.x) {
         const int row = t / a.M, i = t - row * a.M;
         const long r = r0 + row;
@@ -396,7 +442,7 @@ __global__ __launch_bounds__(256) void features_kernel(FeatArgs a) {
             const int f = i * S + s;
             float tv = 0.f;
             if (rs.ok[row]) {
-                tv = (float)((l[f] - z[s]) * rs.D[row][s] / rs.D[row][0]);
+                tv = (float)((l[f] - z[s]) * rs.ts[row][s]);
                 if (a.mu_t) tv = (tv - a.mu_t[f]) / a.sd_t[f];
             }
             tr[f] = tv;
@@ -406,7 +452,7 @@ __global__ __launch_bounds__(256) void features_kernel(FeatArgs a) {
     }
 }
 
-// lambda = lambda_lin + (mu_t + sd_t y) rho0/D_s; NaN where the mean state is
+// lambda = lambda_lin + (mu_t + sd_t y) / ts_s; NaN where the mean state is
 // not physical (the selection then keeps the plain start); zero past a
 // cell's order nt[c] (per-cell orders).  A block covers kRowsPerBlock cells;
 // threads over (cell, moment) pairs.
@@ -432,7 +478,7 @@ __global__ __launch_bounds__(256) void apply_kernel(long n, const double* u, con
                 v = 0.0;  // duals past the cell's order stay zero
             } else if (rs.ok[row]) {
                 const double tv = (double)mu_t[f] + (double)sd_t[f] * (double)y[(size_t)c * NO + f];
-                v = z[s] + tv * rs.D[row][0] / rs.D[row][s];
+                v = z[s] + tv / rs.ts[row][s];
             }
             lam[(size_t)c * M * S + f] = v;
         }
@@ -730,12 +776,11 @@ int launch_features(uqk_mlp* n, long rows, const long* idx, const double* u, con
     a.xh = xh;
     a.xf = xf;
     a.tf = tf;
-    if (n->ctx->kind == UQK_EULER1D)
-        hipLaunchKernelGGL(features_kernel<UQK_EULER1D>, dim3(blocks(rows, kRowsPerBlock)), dim3(256), 0,
-                           n->ctx->stream, a);
-    else
-        hipLaunchKernelGGL(features_kernel<UQK_EULER2D>, dim3(blocks(rows, kRowsPerBlock)), dim3(256), 0,
-                           n->ctx->stream, a);
+    if (!with_warm_kind(n->ctx->kind, [&](auto k) {
+            hipLaunchKernelGGL(features_kernel<decltype(k)::value>, dim3(blocks(rows, kRowsPerBlock)), dim3(256), 0,
+                               n->ctx->stream, a);
+        }))
+        return mfail(n, -1, "warm start: unsupported closure");
     MLP_HIP(n, hipGetLastError());
     return 0;
 }
@@ -759,12 +804,10 @@ int enqueue_predict(uqk_mlp* n, long ncell, const double* u, double* lambda_out,
         if (int rc = launch_features(n, rows, nullptr, uc, nullptr, true, n->act[0], nullptr, nullptr, ntc)) return rc;
         if (int rc = forward(n, rows, false)) return rc;
         double* lc = lambda_out + (size_t)c0 * n->M * n->S;
-        if (n->ctx->kind == UQK_EULER1D)
-            hipLaunchKernelGGL(apply_kernel<UQK_EULER1D>, dim3(blocks(rows, kRowsPerBlock)), dim3(256), 0, st, rows, uc,
-                               ntc, n->y, n->M, n->ctx->c, n->NO, mu_t, sd_t, lc);
-        else
-            hipLaunchKernelGGL(apply_kernel<UQK_EULER2D>, dim3(blocks(rows, kRowsPerBlock)), dim3(256), 0, st, rows, uc,
-                               ntc, n->y, n->M, n->ctx->c, n->NO, mu_t, sd_t, lc);
+        with_warm_kind(n->ctx->kind, [&](auto k) {
+            hipLaunchKernelGGL(apply_kernel<decltype(k)::value>, dim3(blocks(rows, kRowsPerBlock)), dim3(256), 0, st,
+                               rows, uc, ntc, n->y, n->M, n->ctx->c, n->NO, mu_t, sd_t, lc);
+        });
     }
     MLP_HIP(n, hipGetLastError());
     return 0;
@@ -774,8 +817,8 @@ int enqueue_predict(uqk_mlp* n, long ncell, const double* u, double* lambda_out,
 // lambda_alt.
 int enqueue_warm_select(uqk_ctx* ctx, long ncell, const double* u, double* lambda, const double* lambda_alt,
                         const int* active, const int* nt) {
-    if (ctx->kind != UQK_EULER1D && ctx->kind != UQK_EULER2D)
-        return fail_ctx(ctx, -1, "warm start: supported for the Euler closures only");
+    if (!with_warm_kind(ctx->kind, [](auto) {}))
+        return fail_ctx(ctx, -1, "warm start: supported for the Euler and shallow-water closures only");
     unsigned long long* cnt = ctx->d_counters + 4;
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, ctx->stream, cnt);
     // tables once per workgroup, then per wave the points' U values and the duals
@@ -783,12 +826,10 @@ int enqueue_warm_select(uqk_ctx* ctx, long ncell, const double* u, double* lambd
     if (lds > ctx->max_lds) return fail_ctx(ctx, -1, "warm start: selection tables exceed LDS");
     const long want = (ncell + 3) / 4, cap = 8L * (ctx->num_cu > 0 ? ctx->num_cu : 1);
     const dim3 grid((unsigned)(want < cap ? (want > 0 ? want : 1) : cap));
-    if (ctx->kind == UQK_EULER1D)
-        hipLaunchKernelGGL(select_kernel<UQK_EULER1D>, grid, dim3(256), lds, ctx->stream, ctx->c, ncell, ctx->M,
-                           ctx->Q, ctx->d_phi, ctx->d_phiw, u, lambda, lambda_alt, nt, cnt, active);
-    else
-        hipLaunchKernelGGL(select_kernel<UQK_EULER2D>, grid, dim3(256), lds, ctx->stream, ctx->c, ncell, ctx->M,
-                           ctx->Q, ctx->d_phi, ctx->d_phiw, u, lambda, lambda_alt, nt, cnt, active);
+    with_warm_kind(ctx->kind, [&](auto k) {
+        hipLaunchKernelGGL(select_kernel<decltype(k)::value>, grid, dim3(256), lds, ctx->stream, ctx->c, ncell,
+                           ctx->M, ctx->Q, ctx->d_phi, ctx->d_phiw, u, lambda, lambda_alt, nt, cnt, active);
+    });
     UQK_HIP_CTX(ctx, hipGetLastError());
     return 0;
 }
@@ -833,8 +874,8 @@ int uqk_set_warm_start(uqk_ctx* ctx, uqk_mlp* net) {
 int uqk_mlp_create(uqk_ctx* ctx, int32_t hidden, uint64_t seed, uqk_mlp** out) {
     if (!ctx || !out) return -1;
     *out = nullptr;
-    if (ctx->kind != UQK_EULER1D && ctx->kind != UQK_EULER2D)
-        return fail_ctx(ctx, -1, "uqk_mlp_create: the warm start supports the Euler closures");
+    if (!with_warm_kind(ctx->kind, [](auto) {}))
+        return fail_ctx(ctx, -1, "uqk_mlp_create: the warm start supports the Euler and shallow-water closures");
     if (hidden <= 0 || hidden % 32) return fail_ctx(ctx, -1, "uqk_mlp_create: hidden width must be a multiple of 32");
     auto* n = new uqk_mlp();
     n->ctx = ctx;
