@@ -276,14 +276,17 @@ int uqk_set_kernel_timing(uqk_ctx* ctx, int32_t on);
 int uqk_kernel_times(const uqk_ctx* ctx, double* closure_ms, double* advance_ms, int64_t* steps);
 
 /* ---- launch shapes and the autotuner (no counterpart in the reference) --- */
-/* Launch shape of the specialised closure kernel: wavefronts (cells in
- * flight) per workgroup (1, 2, 4; 0 = as many as LDS allows) and persistent
- * workgroups per CU (0 = every resident slot). */
+/* Launch shape of the specialised closure kernels: wavefronts (cells in
+ * flight) per workgroup of the one-wavefront kernel (1, 2, 4; 0 = as many as
+ * LDS allows) and persistent workgroups per CU of either kernel (0 = every
+ * resident slot; the workgroup kernel for n = S*M >= 64 takes only this). */
 int uqk_set_launch_shape(uqk_ctx* ctx, int32_t waves, int32_t blocks_per_cu);
 int uqk_get_launch_shape(const uqk_ctx* ctx, int32_t* waves, int32_t* blocks_per_cu);
-/* Times the specialised closure kernel on n cells (device u and starting
- * lambda; lambda is not modified) for every launch shape the LDS and the
- * occupancy allow, keeps the fastest for ctx, and appends it as one JSON line
+/* Times the specialised closure kernel the context runs (the one-wavefront
+ * kernel, or the workgroup kernel for n = S*M >= 64) on n cells (device u and
+ * starting lambda; lambda is not modified) for every launch shape the LDS and
+ * the occupancy allow, keeps the fastest for ctx, and appends it as one JSON
+ * line
  * {closure, n_states, n_moments, n_quad, precision, waves, blocks_per_cu,
  * ms, cells} to json_path when not NULL.  Returns the number of shapes timed. */
 int uqk_autotune(uqk_ctx* ctx, int64_t n, const double* u, const double* lambda, const char* json_path);

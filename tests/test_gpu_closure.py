@@ -290,6 +290,47 @@ def test_autotuned_launch_shape_gives_identical_duals(tmp_path):
     np.testing.assert_array_equal(out2.download(), ref.download())
 
 
+def test_autotuned_workgroup_kernel(oracle, tmp_path):
+    """The autotuner on the workgroup kernel (the dam-break closure, n = 105,
+    mixed): it sweeps the persistent workgroups per CU, the duals stay
+    bitwise equal to the default launch's, and the JSON line reloads."""
+    b = native.Basis(3, [4], 5)
+    rng = np.random.default_rng(61)
+    c = oracle.closure(oracle.SW2D_BARRIER, eps=1e-8, safe=1, barrier=0.01)
+    n = 2048
+    lam_star = np.zeros((n, b.M, 3))
+    for j in range(n):
+        h, vx, vy = rng.uniform(1, 3), rng.uniform(-1, 1), rng.uniform(-1, 1)
+        lam_star[j, 0] = oracle.DS(c, np.array([h, h * vx, h * vy]))
+        lam_star[j, 1:] = rng.normal(0, 0.02, (b.M - 1, 3)) * np.abs(lam_star[j, 0])
+    u = oracle.reconstruct(3, b.phiw, np.ascontiguousarray(oracle.eval_uq(c, b.phi, lam_star)))
+    lam0 = np.zeros_like(lam_star)
+    for j in range(n):
+        lam0[j, 0] = oracle.DS(c, u[j, 0])
+    du, dl0 = native.DeviceArray.from_host(u), native.DeviceArray.from_host(lam0)
+    kw = dict(barrier=0.01, epsilon=1e-8, safe_line_search=1, precision=native.MIXED)
+    cl = native.GpuClosure(native.SHALLOW_WATER2D_BARRIER, b, **kw)
+    ref = native.DeviceArray.from_host(lam0)
+    assert cl.solve(du, ref) == 0
+    assert cl.last_kernel() == native.KERNEL_BLOCK
+    path = str(tmp_path / "tuning.jsonl")
+    print("case: autotune workgroup kernel SW2D barrier n=105, 2048 cells", flush=True)
+    timed = cl.autotune(du, dl0, path)
+    shape = cl.launch_shape()
+    print("shapes timed", timed, "best", shape, open(path).read(), flush=True)
+    assert timed >= 1 and shape[1] >= 1
+    out = native.DeviceArray.from_host(lam0)
+    assert cl.solve(du, out) == 0
+    assert cl.last_kernel() == native.KERNEL_BLOCK
+    np.testing.assert_array_equal(out.download(), ref.download())
+    cl2 = native.GpuClosure(native.SHALLOW_WATER2D_BARRIER, b, **kw)
+    assert cl2.load_tuning(path) == 1 and cl2.launch_shape() == shape
+    cl2.set_launch_shape(0, 1)  # one workgroup per CU
+    out2 = native.DeviceArray.from_host(lam0)
+    assert cl2.solve(du, out2) == 0
+    np.testing.assert_array_equal(out2.download(), ref.download())
+
+
 @pytest.mark.parametrize("case", ["sw2d_barrier_n105", "euler2d_orders_1_6"])
 def test_mixed_workgroup_kernel(oracle, case):
     """The workgroup kernel's mixed path (Hessian tiles on FP32 MFMA with FP32

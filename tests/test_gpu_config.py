@@ -204,6 +204,75 @@ def test_output_files_and_restart(oracle, tmp_path):
     assert err.max() < 1e-8
 
 
+def test_restart_raises_truncation_order(oracle, tmp_path):
+    """A run at total degree 2 on 3 x 3 points restarted at total degree 3 on
+    5 x 5 points (momentsolver.cpp:418-523): the start state equals the
+    oracle's restatement of the reference's sequence (Dirichlet cells take
+    the initial condition's first moments, DS start converged by the previous
+    closure, Lambda of the previous basis at the new points, U there,
+    projected on the new basis, duals padded with zeros and converged by the
+    new closure), and 5 steps from it match the oracle."""
+    text = open(os.path.join(INPUT, "euler2d_naca.toml")).read()
+    for a, b in [("ni = 512", "ni = 32"), ("nj = 128", "nj = 8"), ("tEnd = 1e3", "tEnd = 2e-5"),
+                 ('moments = [["totalDegree"], [3]]', 'moments = [["totalDegree"], [2]]'),
+                 ('quadPoints = [["tensorizedGrid"], [5]]', 'quadPoints = [["tensorizedGrid"], [3]]')]:
+        assert a in text
+        text = text.replace(a, b)
+    p1 = tmp_path / "run1.toml"
+    p1.write_text(text)
+    pre = str(tmp_path / "out" / "run1")
+    os.makedirs(tmp_path / "out")
+    print("case: restart raising the order, first run (degree 2, 3x3 points)", flush=True)
+    rep1 = native.run_config(str(p1), None, pre)
+    assert rep1["t"] >= 2e-5
+    n, S, M1, M2 = 256, 4, 6, 10
+    u1 = _read_field(pre + "_moments", n, M1, S)
+    fresh = text.replace('moments = [["totalDegree"], [2]]', 'moments = [["totalDegree"], [3]]').replace(
+        'quadPoints = [["tensorizedGrid"], [3]]', 'quadPoints = [["tensorizedGrid"], [5]]').replace(
+        "tEnd = 2e-5", "tEnd = 1e3").replace("maxSteps = 2000", "maxSteps = 5")
+    pf = tmp_path / "fresh.toml"
+    pf.write_text(fresh)
+    _, uic, _, _ = native.config_setup(str(pf))
+    p2 = tmp_path / "run2.toml"
+    p2.write_text(fresh.replace('[general]', '[general]\nrestartFile = "out/run1"'))
+    inf, u0, lam0, _ = native.config_setup(str(p2))
+    assert (inf["M"], inf["Q"]) == (M2, 25)
+    # the oracle's restatement of the start
+    m = meshes.naca_ogrid(32, 8)
+    b1, bi, b2 = native.Basis(2, [2], 3), native.Basis(2, [2], 5), native.Basis(2, [3], 5)
+    c = oracle.closure(oracle.EULER2D, eps=1e-6, safe=1, max_iter=10000)
+    uo = u1.copy()
+    dirichlet = m.bc == native.BC_DIRICHLET
+    assert dirichlet.any()
+    uo[dirichlet] = uic[dirichlet, :M1]
+    lp = np.zeros_like(uo)
+    for j in range(n):
+        lp[j, 0] = oracle.DS(c, uo[j, 0])
+    st, _ = oracle.solve_closure(c, b1.phi, b1.phiw, uo, lp)
+    assert (st == 0).all()
+    u2 = oracle.reconstruct(S, b2.phiw, np.ascontiguousarray(oracle.eval_uq(c, bi.phi, lp)))
+    l2 = np.zeros((n, M2, S))
+    l2[:, :M1] = lp
+    c2 = oracle.closure(oracle.EULER2D, eps=1e-6, safe=1, max_iter=1000)
+    st, _ = oracle.solve_closure(c2, b2.phi, b2.phiw, u2, l2)
+    assert (st == 0).all()
+    du = np.abs(u0 - u2).max() / np.abs(u2).max()
+    dl = (np.abs(lam0 - l2) / np.abs(l2).max(axis=1, keepdims=True)).max()
+    print("start state vs oracle: u", du, "lambda", dl, flush=True)
+    assert du < 1e-9 and dl < 1e-7
+    csv2 = str(tmp_path / "mv2.csv")
+    rep2 = native.run_config(str(p2), csv2)
+    assert rep2["steps"] == 5 and rep2["t"] > 2e-5
+    co = _oracle_closure(oracle, native.EULER2D, eps=1e-6)
+    oprob = oracle.problem(oracle.P_EULER2D, flux=oracle.FLUX_LF, cfl=0.4)
+    uo2, lo2 = u2.copy(), l2.copy()
+    res = oracle.run(co, oprob, _oracle_mesh(oracle, m), b2.phi, b2.phiw, b2.xi, uo2, lo2, 1e3, 5)
+    assert res["rc"] == 0 and res["steps"] == 5
+    err = _rel_l2(np.loadtxt(csv2, delimiter=",", ndmin=2), oracle.mean_var(co, b2.phi, b2.phiw, lo2), m.area)
+    print("raised-order restart rel L2", err, flush=True)
+    assert err.max() < 1e-7
+
+
 def test_output_files_1d(tmp_path):
     """Burgers: the strip of quadrilaterals (two rows of N+1 points, a tenth of
     the domain high) and the ExpectedValue / Variance text files"""

@@ -661,16 +661,26 @@ size_t newton_block_lds(int S, int M, int Q) {
 }
 
 template <int KIND>
-static hipError_t block_go(const NewtonArgs& args, int max_lds, int num_cu, hipStream_t st) {
+static int block_occupancy(int M, int Q, int prec) {
+    constexpr int S = KindInfo<KIND>::S;
+    const size_t shmem = newton_block_lds(S, M, Q);
+    int occ = 0;
+    const hipError_t e =
+        prec == UQK_FP64
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, newton_block_kernel<KIND, 0>, kBlockThreads, shmem)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, newton_block_kernel<KIND, 1>, kBlockThreads, shmem);
+    return e == hipSuccess && occ > 0 ? occ : 1;
+}
+
+template <int KIND>
+static hipError_t block_go(const NewtonArgs& args, int max_lds, int num_cu, hipStream_t st, const LaunchShape& shape) {
     constexpr int S = KindInfo<KIND>::S;
     const size_t shmem = newton_block_lds(S, args.M, args.Q);
     if (shmem > (size_t)max_lds || S * args.M > kBlockMaxN || args.Q > kBlockThreads) return hipErrorInvalidValue;
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, newton_block_kernel<KIND, 0>, kBlockThreads, shmem) !=
-            hipSuccess ||
-        occ <= 0)
-        occ = 1;
-    long blocks = args.ncell < (long)occ * num_cu ? args.ncell : (long)occ * num_cu;
+    const int occ = block_occupancy<KIND>(args.M, args.Q, args.prec);
+    // persistent workgroups per CU: the resident maximum, or the tuned count
+    const int per_cu = shape.blocks_per_cu > 0 && shape.blocks_per_cu < occ ? shape.blocks_per_cu : occ;
+    long blocks = args.ncell < (long)per_cu * num_cu ? args.ncell : (long)per_cu * num_cu;
     if (blocks < 1) blocks = 1;
     NewtonArgs a = args;
     // the queue counts workgroups here: one cell per workgroup at a time
@@ -686,15 +696,27 @@ bool newton_block_fits(int S, int M, int Q, int max_lds) {
     return S * M <= kBlockMaxN && Q <= kBlockThreads && newton_block_lds(S, M, Q) <= (size_t)max_lds;
 }
 
-hipError_t newton_block_launch(int kind, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st) {
+hipError_t newton_block_launch(int kind, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
+                               const LaunchShape& shape) {
     switch (kind) {
-        case UQK_BOUNDED_BARRIER: return block_go<UQK_BOUNDED_BARRIER>(a, max_lds, num_cu, st);
-        case UQK_EULER1D: return block_go<UQK_EULER1D>(a, max_lds, num_cu, st);
-        case UQK_EULER2D: return block_go<UQK_EULER2D>(a, max_lds, num_cu, st);
-        case UQK_SHALLOW_WATER2D: return block_go<UQK_SHALLOW_WATER2D>(a, max_lds, num_cu, st);
-        case UQK_SHALLOW_WATER2D_BARRIER: return block_go<UQK_SHALLOW_WATER2D_BARRIER>(a, max_lds, num_cu, st);
+        case UQK_BOUNDED_BARRIER: return block_go<UQK_BOUNDED_BARRIER>(a, max_lds, num_cu, st, shape);
+        case UQK_EULER1D: return block_go<UQK_EULER1D>(a, max_lds, num_cu, st, shape);
+        case UQK_EULER2D: return block_go<UQK_EULER2D>(a, max_lds, num_cu, st, shape);
+        case UQK_SHALLOW_WATER2D: return block_go<UQK_SHALLOW_WATER2D>(a, max_lds, num_cu, st, shape);
+        case UQK_SHALLOW_WATER2D_BARRIER: return block_go<UQK_SHALLOW_WATER2D_BARRIER>(a, max_lds, num_cu, st, shape);
     }
     return hipErrorInvalidValue;
+}
+
+int newton_block_occupancy(int kind, int M, int Q, int prec) {
+    switch (kind) {
+        case UQK_BOUNDED_BARRIER: return block_occupancy<UQK_BOUNDED_BARRIER>(M, Q, prec);
+        case UQK_EULER1D: return block_occupancy<UQK_EULER1D>(M, Q, prec);
+        case UQK_EULER2D: return block_occupancy<UQK_EULER2D>(M, Q, prec);
+        case UQK_SHALLOW_WATER2D: return block_occupancy<UQK_SHALLOW_WATER2D>(M, Q, prec);
+        case UQK_SHALLOW_WATER2D_BARRIER: return block_occupancy<UQK_SHALLOW_WATER2D_BARRIER>(M, Q, prec);
+    }
+    return 1;
 }
 
 }  // namespace uqk

@@ -247,7 +247,7 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         a.uq_out = fuse ? uq_out : nullptr;
         a.u_out = fuse ? u_out : nullptr;
         if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev0, ctx->stream));
-        UQK_HIP(ctx, uqk::newton_block_launch(ctx->kind, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream));
+        UQK_HIP(ctx, uqk::newton_block_launch(ctx->kind, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream, ctx->shape));
         if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev1, ctx->stream));
         ctx->last_kernel = UQK_KERNEL_BLOCK;
         ctx->timed = ev;
@@ -969,21 +969,32 @@ int uqk_get_launch_shape(const uqk_ctx* ctx, int32_t* waves, int32_t* blocks_per
 
 int uqk_autotune(uqk_ctx* ctx, int64_t n, const double* u, const double* lambda, const char* json_path) {
     if (!ctx || n <= 0 || !u || !lambda) return -1;
+    // the kernel the context runs: the one-wavefront kernel where it fits,
+    // else the workgroup kernel (n = S*M >= 64), whose only launch parameter
+    // is the number of persistent workgroups per CU (one wave count: 4)
     int wmax = 0, occ = 0;
-    if (!uqk::newton_wave_limits(ctx->kind, ctx->M, ctx->Q, ctx->precision, (int)ctx->max_lds, 4, &wmax, &occ))
-        return fail(ctx, -3, "uqk_autotune: no specialised closure kernel for this shape");
+    const bool wave = ctx->kernel != UQK_KERNEL_BLOCK &&
+                      uqk::newton_wave_limits(ctx->kind, ctx->M, ctx->Q, ctx->precision, (int)ctx->max_lds, 4, &wmax,
+                                              &occ);
+    const bool block = !wave && (ctx->kernel == UQK_KERNEL_AUTO || ctx->kernel == UQK_KERNEL_BLOCK) &&
+                       uqk::newton_block_fits(ctx->S, ctx->M, ctx->Q, (int)ctx->max_lds);
+    if (!wave && !block) return fail(ctx, -3, "uqk_autotune: no specialised closure kernel for this shape");
+    if (block) wmax = 4;
     const size_t bytes = sizeof(double) * (size_t)n * ctx->S * ctx->M;
     double* trial = nullptr;
     UQK_HIP(ctx, hipMalloc(&trial, bytes));
     const uqk::LaunchShape keep = ctx->shape;
     const int kernel_keep = ctx->kernel;
-    ctx->kernel = UQK_KERNEL_WAVE;
+    ctx->kernel = wave ? UQK_KERNEL_WAVE : UQK_KERNEL_BLOCK;
     double best_ms = -1.0;
     uqk::LaunchShape best = keep;
     int timed = 0, rc = 0;
-    for (int waves = 1; waves <= wmax && rc == 0; waves *= 2) {
+    for (int waves = block ? 4 : 1; waves <= wmax && rc == 0; waves *= 2) {
         int wm = 0, oc = 0;
-        uqk::newton_wave_limits(ctx->kind, ctx->M, ctx->Q, ctx->precision, (int)ctx->max_lds, waves, &wm, &oc);
+        if (wave)
+            uqk::newton_wave_limits(ctx->kind, ctx->M, ctx->Q, ctx->precision, (int)ctx->max_lds, waves, &wm, &oc);
+        else
+            oc = uqk::newton_block_occupancy(ctx->kind, ctx->M, ctx->Q, ctx->precision);
         for (int per_cu = 1; per_cu <= oc && rc == 0; ++per_cu) {
             ctx->shape.waves = waves;
             ctx->shape.blocks_per_cu = per_cu;

@@ -120,7 +120,11 @@ bool newton_wave_limits(int kind, int M, int Q, int prec, int max_lds, int waves
 int newton_fp64_wave_doubles(int S, int M, int Q);
 // workgroup-per-cell kernel for n = S*M up to 256 (closure_block.hip)
 bool newton_block_fits(int S, int M, int Q, int max_lds);
-hipError_t newton_block_launch(int kind, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st);
+// shape.blocks_per_cu > 0 caps the persistent workgroups per CU
+hipError_t newton_block_launch(int kind, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
+                               const LaunchShape& shape = LaunchShape());
+// resident workgroups per CU of the workgroup kernel for (kind, M, Q, prec)
+int newton_block_occupancy(int kind, int M, int Q, int prec);
 int newton_fp64_occupancy(int kind, bool tlds, int threads, size_t shmem);
 hipError_t newton_fp64_launch(int kind, const NewtonArgs& a, bool tlds, int blocks, int threads, size_t shmem,
                               hipStream_t st);

@@ -180,13 +180,98 @@ struct RunOutput {
 // DeterminePreviousSettings, SetDuals; momentsolver.cpp:418-523): the
 // moments of <restartFile>_moments, the duals of <restartFile>_duals when
 // importDualState is set, and the previous run's tEnd as the start time.
-// A restart keeps the previous basis and quadrature (the reference can also
-// raise the order on restart; that path is not provided here).
+// When the previous run used other moments or another quadrature,
+// restart_other_basis below maps its state onto the current basis.
 struct Start {
     std::vector<double> u, lam;
     double t0 = 0.0;
     bool presolve = false;
 };
+
+// A restart whose previous run used other moments or another quadrature
+// (DetermineMoments, SetDuals; momentsolver.cpp:418-523), on the GPU:
+//  1. the previous moments (its M_p per cell); when M_p differs from M,
+//     Dirichlet cells take the initial condition at the current order, of
+//     which the previous closure reads the first M_p moments (:423-429);
+//  2. the previous duals when importDualState is set, else (DS(u_0), 0, ...)
+//     converged by the previous closure (its basis and quadrature, 10000
+//     iterations, SolveClosureSafe) when the moments changed or the closure is
+//     one-shot (:466-493);
+//  3. when the moments changed (:495-521): Lambda of the previous basis at
+//     the current quadrature points, U there, projected on the current basis
+//     gives the moments; the duals are the previous ones padded with zeros
+//     (or cut) to M, converged by the current closure (SolveClosureSafe;
+//     10000 iterations for a one-shot closure).
+// The start-up closures run in FP64 whatever the run's precision.
+static void restart_other_basis(const Settings& s, const Settings& prev, const Mesh& m, const Basis& b,
+                                const std::string& rf, Start& st) {
+    const long N = m.n_cells;
+    const int S = s.n_states, M = b.M;
+    Settings ps = s;  // DeterminePreviousSettings: the previous basis, the current closure
+    ps.degrees = prev.degrees;
+    ps.max_degree = prev.max_degree;
+    ps.quad_points = prev.quad_points;
+    ps.precision = UQK_FP64;
+    Settings cs = s;
+    cs.precision = UQK_FP64;
+    const Basis bp = make_basis(ps);
+    const int Mp = bp.M;
+    const bool moments_changed = prev.degrees != s.degrees || prev.max_degree != s.max_degree;
+    std::vector<double> u = read_restart_field(rf + "_moments", N, S, Mp);
+    if (Mp != M) {
+        const std::vector<double> uic = setup_ic(s, m, b);
+        for (long j = 0; j < N; ++j)
+            if (m.bc[j] == UQK_BC_DIRICHLET)
+                for (int i = 0; i < std::min(Mp, M); ++i)
+                    for (int l = 0; l < S; ++l) u[(size_t)j * S * Mp + i * S + l] = uic[(size_t)j * S * M + i * S + l];
+    }
+    std::vector<double> lam;
+    if (s.import_dual_state) {
+        lam = read_restart_field(rf + "_duals", N, S, Mp);
+    } else {
+        lam = set_duals(s, u, N, Mp);
+        if (s.max_iterations == 1 || moments_changed) {
+            std::unique_ptr<uqk_ctx, int (*)(uqk_ctx*)> cp(make_ctx(ps, bp, 10000, true), uqk_destroy);
+            DeviceBuf du(u.data(), sizeof(double) * u.size()), dl(lam.data(), sizeof(double) * lam.size());
+            if (uqk_closure_solve(cp.get(), N, nullptr, du.as<double>(), dl.as<double>(), nullptr, nullptr) < 0)
+                throw std::runtime_error(uqk_last_error(cp.get()));
+            if (uqk_memcpy(lam.data(), dl.p, (int64_t)(sizeof(double) * lam.size()), 1) != 0)
+                throw std::runtime_error("device to host copy failed");
+        }
+    }
+    if (!moments_changed) {  // another quadrature only: the same moments and duals
+        st.u = std::move(u);
+        st.lam = std::move(lam);
+        return;
+    }
+    Basis bi;  // the previous index set on the current quadrature
+    bi.idx = bp.idx;
+    if (!basis_tables(s.dim_xi, s.quad_points, bi.idx, bi.xi, bi.wq, bi.phi, bi.phiw))
+        throw std::runtime_error("quadrature failed");
+    bi.M = Mp;
+    bi.Q = b.Q;
+    std::unique_ptr<uqk_ctx, int (*)(uqk_ctx*)> ci(make_ctx(cs, bi, 1, true), uqk_destroy);
+    std::unique_ptr<uqk_ctx, int (*)(uqk_ctx*)> cn(
+        make_ctx(cs, b, s.max_iterations == 1 ? 10000 : s.max_iterations, true), uqk_destroy);
+    DeviceBuf dlp(lam.data(), sizeof(double) * lam.size());
+    DeviceBuf duq(sizeof(double) * (size_t)N * b.Q * S), dun(sizeof(double) * (size_t)N * M * S);
+    if (uqk_eval_uq(ci.get(), N, nullptr, dlp.as<double>(), duq.as<double>()) != 0)
+        throw std::runtime_error(uqk_last_error(ci.get()));
+    if (uqk_reconstruct(cn.get(), N, nullptr, duq.as<double>(), dun.as<double>()) != 0)
+        throw std::runtime_error(uqk_last_error(cn.get()));
+    std::vector<double> ln((size_t)N * M * S, 0.0);
+    for (long j = 0; j < N; ++j)
+        for (int i = 0; i < std::min(Mp, M); ++i)
+            for (int l = 0; l < S; ++l) ln[(size_t)j * S * M + i * S + l] = lam[(size_t)j * S * Mp + i * S + l];
+    DeviceBuf dln(ln.data(), sizeof(double) * ln.size());
+    if (uqk_closure_solve(cn.get(), N, nullptr, dun.as<double>(), dln.as<double>(), nullptr, nullptr) < 0)
+        throw std::runtime_error(uqk_last_error(cn.get()));
+    st.u.assign((size_t)N * M * S, 0.0);
+    if (uqk_memcpy(st.u.data(), dun.p, (int64_t)(sizeof(double) * st.u.size()), 1) != 0 ||
+        uqk_memcpy(ln.data(), dln.p, (int64_t)(sizeof(double) * ln.size()), 1) != 0)
+        throw std::runtime_error("device to host copy failed");
+    st.lam = std::move(ln);
+}
 
 static Start start_of(const Settings& s, const Mesh& m, const Basis& b) {
     Start st;
@@ -200,9 +285,13 @@ static Start start_of(const Settings& s, const Mesh& m, const Basis& b) {
     }
     const std::string rf = s.restart_file[0] == '/' ? s.restart_file : s.input_dir + "/" + s.restart_file;
     const Settings prev = Settings::from_toml(Toml::parse(read_log_config(rf)));
-    if (prev.degrees != s.degrees || prev.max_degree != s.max_degree || prev.quad_points != s.quad_points ||
-        prev.dim_xi != s.dim_xi)
-        throw std::runtime_error("restart: the previous run used other moments or quadrature (not supported)");
+    if (prev.dim_xi != s.dim_xi)
+        throw std::runtime_error("restart: the previous run had another number of uncertain dimensions");
+    if (prev.degrees != s.degrees || prev.max_degree != s.max_degree || prev.quad_points != s.quad_points) {
+        restart_other_basis(s, prev, m, b, rf, st);
+        st.t0 = prev.t_end;
+        return st;
+    }
     st.u = read_restart_field(rf + "_moments", N, S, M);
     st.lam = s.import_dual_state ? read_restart_field(rf + "_duals", N, S, M) : set_duals(s, st.u, N, M);
     st.t0 = prev.t_end;
