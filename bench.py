@@ -40,35 +40,11 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# dense peaks of one MI355X
-FP64_PEAK_TFLOPS = 78.6
-FP32_PEAK_TFLOPS = 157.3
-HBM_PEAK_GBS = 8000.0
+from uqcreator_amd.roofline import advance_roofline, closure_roofline, flop_model, rel_l2  # noqa: E402
 
 C3_NI, C3_NJ = 512, 128
 C3_EPS = 1e-6
 C3_CFL = 0.4
-
-
-def flop_model(S, M, Q):
-    """Algorithmic flops of one factored Newton step (symmetric Hessian
-    Q n (n+1), Cholesky + solve n^3/3 + 2 n^2), of the inverse factor the
-    mixed closure keeps after it (triangular inverse n^3/3), of one step on a
-    kept inverse (two applications of V V^T, 4 n^2, and the FP64
-    Hessian-vector product 4 Q S M + 2 Q S^2) and of one gradient evaluation
-    (4 Q S M)."""
-    n = S * M
-    return dict(F_H=Q * n * (n + 1), F_C=n ** 3 / 3.0 + 2.0 * n * n, F_inv=n ** 3 / 3.0,
-                F_lag=4.0 * n * n + 4.0 * Q * S * M + 2.0 * Q * S * S, F_g=4.0 * Q * S * M)
-
-
-def advance_bytes_per_cell(S, M, Q, edges, dim):
-    """Algorithmic HBM bytes of the fused advance per cell: uQ of the cell
-    (the neighbours' uQ are the same array, read again through L2), the
-    reconstructed moments, the new moments, and the cell's geometry
-    (neighbour ids, unit and scaled normals, area, boundary type and normal)."""
-    geometry = edges * 4 + 2 * edges * dim * 8 + 8 + 4 + dim * 8
-    return 8 * S * Q + 8 * S * M + 8 * S * M + geometry
 
 
 def cpu_threads():
@@ -78,42 +54,6 @@ def cpu_threads():
     except AttributeError:
         n = os.cpu_count() or 1
     return max(1, min(16, n))
-
-
-def closure_roofline(kernel, kernel_ms, newton, grads, fm, precision_mixed, factors=None, lagged=0.0):
-    """Roofline of a closure kernel: algorithmic flops per launch over its
-    HIP-event time; the peak is time-weighted over the units the flops run
-    on (Hessian on FP64 or FP32 MFMA, the rest FP64).  factors / lagged:
-    factored Newton steps and steps on a kept inverse (mixed closure); by
-    default every Newton step is factored."""
-    if factors is None:
-        factors = newton
-    f_hess = factors * fm["F_H"]
-    f_fp64 = factors * fm["F_C"] + grads * fm["F_g"]
-    if lagged or (precision_mixed and factors != newton):
-        f_fp64 += factors * fm["F_inv"] + lagged * fm["F_lag"]
-    flops = f_hess + f_fp64
-    hess_peak = FP32_PEAK_TFLOPS if precision_mixed else FP64_PEAK_TFLOPS
-    t_min = f_hess / (hess_peak * 1e12) + f_fp64 / (FP64_PEAK_TFLOPS * 1e12)
-    achieved = flops / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
-    peak = flops / t_min / 1e12 if t_min > 0 else FP64_PEAK_TFLOPS
-    return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-            "traffic": None, "kernel": kernel, "kernel_ms": kernel_ms, "flops_per_launch": flops,
-            "hessian_flops_per_launch": f_hess, "newton_steps_per_launch": newton,
-            "factorizations_per_launch": factors, "lagged_steps_per_launch": lagged,
-            "gradient_evals_per_launch": grads, "flop_model": fm}
-
-
-def rel_l2(a, b, area=None):
-    """Area-weighted relative L2 per field (momentsolver.cpp:687-749), the
-    largest over the fields; a field whose norm is below 1e-10 of the largest
-    is measured against that floor."""
-    import numpy as np
-
-    w = np.ones(a.shape[0]) if area is None else area
-    num = ((a - b) ** 2 * w[:, None]).sum(0)
-    den = (b ** 2 * w[:, None]).sum(0)
-    return float(np.sqrt(num / np.maximum(den, 1e-20 * den.max())).max())
 
 
 # --------------------------------------------------------------------------
@@ -189,12 +129,8 @@ class C3:
                  "newton_wave_kernel<2,10,25,%d>" % precision)
         roof_c = closure_roofline(kname, closure_ms / timed_steps, newton / args.steps, grads / args.steps, fm,
                                   precision == native.MIXED, factors / args.steps, lagged / args.steps)
-        bpc = advance_bytes_per_cell(S, M, Q, self.m.nbr.shape[1], 2)
         adv_ms = advance_ms / timed_steps
-        adv_gbs = bpc * self.n / (adv_ms * 1e-3) / 1e9
-        roof_a = {"bound": "hbm", "achieved": adv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": adv_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "advance_kernel<2>",
-                  "kernel_ms": adv_ms, "bytes_per_launch": bpc * self.n, "bytes_per_cell": bpc}
+        roof_a = advance_roofline("advance_kernel<2>", adv_ms, S, M, Q, self.m.nbr.shape[1], 2, self.n)
         out = {
             "value": self.n * args.steps / elapsed,
             "ms_per_step": 1e3 * elapsed / args.steps,
@@ -305,7 +241,7 @@ def c2_leg(native, np, b, u, lam0, lam_star, args, precision, warm=None):
     dmv = native.DeviceArray((args.c2_cells, 6))
     cl.mean_var(lam, dmv)
     fm = flop_model(3, b.M, b.Q)
-    kname = "newton_wave_kernel<1,15,49,%d>" % precision
+    kname = "newton_wave_kernel<1,15,49,%d>" % (4 if lagged > 0 else precision)
     leg = {
         "value": args.c2_cells / (elapsed / args.c2_steps),
         "ms_per_step": 1e3 * elapsed / args.c2_steps,
@@ -317,7 +253,7 @@ def c2_leg(native, np, b, u, lam0, lam_star, args, precision, warm=None):
         "lambda_rel_err_vs_known": float((np.abs(lam_h - lam_star) / scale).max()),
         "roofline": closure_roofline(kname, float(np.mean(kernel_ms)), newton / args.c2_steps,
                                      grads / args.c2_steps, fm, precision == native.MIXED, factors / args.c2_steps,
-                                     lagged / args.c2_steps),
+                                     lagged / args.c2_steps, reuse="factor"),
     }
     if net is not None:
         net.detach()
