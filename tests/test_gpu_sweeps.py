@@ -272,3 +272,45 @@ def test_mixed_factor_reuse_time_loop(oracle):
     assert got[False][1] == 0 and got[False][0] == got[False][2]
     assert got[True][0] + got[True][1] <= got[True][2]  # (a rejected lagged step counts as a Newton step)
     assert got[True][0] < 0.5 * got[False][0]
+
+
+@pytest.mark.parametrize("precision", [native.FP64, native.MIXED])
+def test_residual_check_leaves_results_unchanged(precision):
+    """The residual check before the Newton kernel (one lane per cell) takes
+    Lambda and U at the points as the Newton kernel does and sums the
+    projection in point order: on the NACA0012 O-grid, 40 steps with the
+    check on and off end converged everywhere with E/Var fields equal to
+    1e-12 and nearly the same Newton steps; with the check, the captured
+    graph equals the host-enqueued loop bitwise."""
+    b, m, u0 = workloads.euler2d_c3(128, 32)
+    lam0 = np.zeros_like(u0)
+    lam0[:, 0] = workloads.euler2d_ds(u0[:, 0])
+    prob = native.problem(native.EULER2D_PROBLEM, flux=native.FLUX_LAX_FRIEDRICHS, cfl=0.4)
+    out = {}
+    for check in (False, True):
+        for graph in (False, True):
+            print("case: residual check", check, "graph", graph, "precision", precision, flush=True)
+            cl = native.GpuClosure(native.EULER2D, b, epsilon=1e-6, precision=precision)
+            cl.set_residual_check(check)
+            s = solver.GpuMomentSolver(cl, prob, m)
+            s.set_state(u0, lam0)
+            s.run_device(1e3, max_steps=40, use_graph=graph)
+            assert s.steps == 40
+            out[(check, graph)] = (s.lam.download(), s.u.download(), s.status.download(), s.iters.download(),
+                                   s.mean_var(), s.newton_steps, s.t)
+    for check in (False, True):
+        host, graph = out[(check, False)], out[(check, True)]
+        for x, y in zip(host[:5], graph[:5]):
+            np.testing.assert_array_equal(x, y)
+        assert host[5:] == graph[5:]
+        assert (host[2] == 0).all()
+    off, on = out[(False, False)], out[(True, False)]
+    conv = int((on[3] == 0).sum())
+    print("cells finished by the check in the last step:", conv, "of", m.n_cells, "Newton steps on/off", on[5], off[5],
+          flush=True)
+    assert 0 < conv < m.n_cells
+    err = _rel_l2(on[4], off[4], m.area)
+    print("E/Var rel L2 check on vs off", err, flush=True)
+    assert err.max() < 1e-12
+    assert abs(on[5] - off[5]) <= 0.005 * off[5]
+    assert on[6] == pytest.approx(off[6], rel=1e-12)

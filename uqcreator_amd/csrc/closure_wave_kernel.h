@@ -152,11 +152,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
     // with one atomicAdd, so chunks start large (few atomics) and shrink to
     // single positions at the end (balance).  Position t of the 2^qbits queue
     // is cell bitreverse(t) (skipped when >= ncell): neighbouring costly cells
-    // go to different chunks.  Newton and failure counts are summed per wave
-    // (wave-uniform, scalar) and added once at the end.
+    // go to different chunks.  After a residual check the queue runs over
+    // the *cell_count listed cells instead: position t is
+    // cell_list[bitreverse(t)] (skipped past the count).  Newton and failure
+    // counts are summed per wave (wave-uniform, scalar) and added once at the
+    // end.
     unsigned long long w_fail = 0, w_newton = 0, w_grad = 0, w_took = 0, w_factor = 0, w_reuse = 0;
     double w_dt = 1e10;  // min ComputeDt over this wave's cells (fused path)
-    const long qsize = 1L << a.qbits;
+    const long nlist = a.cell_list ? (long)__builtin_amdgcn_readfirstlane((int)*a.cell_count) : 0;
+    int lbits = 0;
+    while ((1L << lbits) < nlist) ++lbits;
+    const int qb = a.cell_list ? lbits : a.qbits;
+    const long qsize = 1L << qb;
     long qpos = 0, qend = 0;
     for (;;) {
         if (qpos == qend) {
@@ -172,7 +179,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
         }
         const unsigned pos = (unsigned)qpos;
         ++qpos;
-        const long cell = a.qbits ? (long)(__builtin_bitreverse32(pos) >> (32 - a.qbits)) : 0;
+        const long qcell = qb ? (long)(__builtin_bitreverse32(pos) >> (32 - qb)) : 0;
+        long cell = qcell;
+        if (a.cell_list) {
+            if (qcell >= nlist) continue;
+            cell = (long)__builtin_amdgcn_readfirstlane(a.cell_list[qcell]);
+        }
         if (cell >= a.ncell) continue;
         const double* ug = a.u + cell * SM;
         double* lg = a.lam + cell * SM;

@@ -129,6 +129,19 @@ int ensure_factor_buffers(uqk_ctx* ctx, long n_cells) {
     return 0;
 }
 
+// The list of cells the residual check leaves to the Newton kernel: grown to
+// n cells, outside graph capture only.
+int ensure_check_list(uqk_ctx* ctx, long n_cells) {
+    if (ctx->check_cap >= n_cells) return 0;
+    if (ctx->capturing) return fail(ctx, -1, "residual check: the cell list must exist before graph capture");
+    hipFree(ctx->check_list);
+    ctx->check_list = nullptr;
+    ctx->check_cap = 0;
+    UQK_HIP(ctx, hipMalloc(&ctx->check_list, sizeof(int) * (size_t)(n_cells > 0 ? n_cells : 1)));
+    ctx->check_cap = n_cells;
+    return 0;
+}
+
 // Launch the Newton closure kernel; leaves the failed-cell count in counters[1]
 // (counters: the context's d_counters unless the caller brings its own
 // kClosureCounters).
@@ -217,6 +230,22 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         a.u_out = fuse ? u_out : nullptr;
         a.lam_alt = lam_alt;
         if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev0, ctx->stream));
+        // the time loop's step: the residual check finishes the cells whose
+        // lambda already passes the convergence test (not SolveClosureSafe,
+        // which always takes a step) and lists the rest for the wave kernel
+        int wm = 0, wo = 0;
+        if (fuse && ctx->residual_check && !ctx->c.safe &&
+            uqk::newton_wave_limits(ctx->kind, ctx->M, ctx->Q, ctx->precision, (int)ctx->max_lds, 1, &wm, &wo)) {
+            if (int rc = ensure_check_list(ctx, n)) return rc;
+            uqk::NewtonArgs ca = a;
+            ca.list_out = ctx->check_list;
+            ca.list_count = counters + 8;
+            if (uqk::closure_check_launch(ctx->kind, ctx->M, ctx->Q, ca, ctx->num_cu, ctx->stream, &e)) {
+                UQK_HIP(ctx, e);
+                a.cell_list = ctx->check_list;
+                a.cell_count = counters + 8;
+            }
+        }
         if (uqk::newton_wave_launch(ctx->kind, ctx->M, ctx->Q, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream, &e,
                                     ctx->shape)) {
             UQK_HIP(ctx, e);
@@ -511,6 +540,7 @@ int uqk_destroy(uqk_ctx* ctx) {
     hipFree(ctx->ufac_cell);
     hipFree(ctx->pinv_ok);
     hipFree(ctx->pinv_list);
+    hipFree(ctx->check_list);
     if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
     hipFree(ctx->d_run);
     if (ctx->h_run) hipHostFree(ctx->h_run);
@@ -798,6 +828,8 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
     if (int rc = uqk::prepare_warm_start(ctx, mesh->n_cells)) return rc;
     if (ctx->precision == UQK_MIXED && ctx->factor_reuse && !nt && mesh->n_cells > 0)
         if (int rc = ensure_factor_buffers(ctx, mesh->n_cells)) return rc;
+    if (ctx->residual_check && !nt && mesh->n_cells > 0)
+        if (int rc = ensure_check_list(ctx, mesh->n_cells)) return rc;
     UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *ctx->h_run = uqk::RunState{};
     ctx->h_run->t = t0;
@@ -915,6 +947,13 @@ int uqk_set_kernel_timing(uqk_ctx* ctx, int32_t on) {
     ctx->ktiming = on != 0;
     ctx->k_closure_ms = ctx->k_advance_ms = 0.0;
     ctx->k_steps = 0;
+    return 0;
+}
+
+int uqk_set_residual_check(uqk_ctx* ctx, int32_t on) {
+    if (!ctx) return -1;
+    if (ctx->residual_check != (on != 0)) uqk::invalidate_graph(ctx);
+    ctx->residual_check = on != 0;
     return 0;
 }
 

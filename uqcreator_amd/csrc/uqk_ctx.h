@@ -66,6 +66,11 @@ struct uqk_ctx {
     double* ufac_cell = nullptr;
     int* pinv_ok = nullptr;
     int* pinv_list = nullptr;  // cells with a new factor, per launch
+    // residual check before the wave kernel in the time loop (on by
+    // default): the cells it leaves to the Newton kernel
+    bool residual_check = true;
+    int* check_list = nullptr;
+    long check_cap = 0;
     long pinv_cell_cap = 0;
     std::string err;
 };

@@ -43,6 +43,14 @@ struct NewtonArgs {
     // wins); *took += cells that started from lam_alt
     const double* lam_alt;
     unsigned long long* took;
+    // residual check before the Newton kernel (time loop): the check kernel
+    // finishes every cell that passes ||grad L|| < eps at its lambda and
+    // appends the others to list_out[(*list_count)++]; the wave kernel then
+    // takes its cells from cell_list[0 .. *cell_count) (null: every cell)
+    int* list_out;
+    unsigned long long* list_count;
+    const int* cell_list;
+    const unsigned long long* cell_count;
     // mixed precision, wave kernel: reuse of the factorisation.
     // pinv_mode 1 (time loop, across steps): a cell whose pinv_ok is 1 holds
     // a kept inverse Hessian in pinv (FP32, n x n, row stride n + 1) and
@@ -85,8 +93,9 @@ struct RunState {
 // Closure counters of a launch: [0] work-queue head, [1] failed cells, [2]
 // Newton steps, [3] gradient evaluations, [4] cells that started from the
 // learned warm start, [5] Cholesky factorisations, [6] steps on a kept
-// inverse, [7] cells listed for factor_inverse_kernel; all zero at launch.
-constexpr int kClosureCounters = 8;
+// inverse, [7] cells listed for factor_inverse_kernel, [8] cells the
+// residual check left to the Newton kernel; all zero at launch.
+constexpr int kClosureCounters = 9;
 
 // Launch shape of the specialised closure kernel; 0 = automatic (as many
 // waves per workgroup as LDS allows, every resident workgroup slot filled).
@@ -105,6 +114,10 @@ inline void set_queue(NewtonArgs& a, long resident_waves) {
 
 bool newton_wave_launch(int kind, int M, int Q, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st,
                         hipError_t* err, const LaunchShape& shape = LaunchShape());
+// Residual check of the wave kernel's shapes (closure_check.hip): a.list_out
+// and a.list_count receive the cells left to the Newton kernel; false when
+// the shape has no check kernel.
+bool closure_check_launch(int kind, int M, int Q, const NewtonArgs& a, int num_cu, hipStream_t st, hipError_t* err);
 // Whether (kind, M, Q) has the mixed wave kernel with kept inverses.
 bool newton_wave_reuse_fits(int kind, int M, int Q, int max_lds);
 // Inverses of the factors the last mixed wave launch stored and listed

@@ -102,6 +102,7 @@ SIGNATURES = {
     "uqk_last_closure_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "uqk_set_kernel_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "uqk_set_factor_reuse": (C.c_int, [C.c_void_p, C.c_int32]),
+    "uqk_set_residual_check": (C.c_int, [C.c_void_p, C.c_int32]),
     "uqk_invalidate_factors": (C.c_int, [C.c_void_p]),
     "uqk_last_closure_factorizations": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "uqk_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
@@ -445,6 +446,11 @@ class GpuClosure:
         """Mixed closure: keep each cell's inverse Hessian and take refined
         steps on it until the contraction weakens (on by default for MIXED)."""
         self._check(lib().uqk_set_factor_reuse(self.h, int(bool(on))), "uqk_set_factor_reuse")
+
+    def set_residual_check(self, on):
+        """Residual check before the Newton kernel in the time loop (on by
+        default): finishes the cells already converged, lists the others."""
+        self._check(lib().uqk_set_residual_check(self.h, int(bool(on))), "uqk_set_residual_check")
 
     def invalidate_factors(self):
         """Forget the kept inverses (the next closure factors every cell)."""
