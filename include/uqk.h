@@ -263,14 +263,15 @@ int uqk_last_closure_stats(const uqk_ctx* ctx, int64_t* newton_steps, int64_t* g
  * inverses (after replacing the duals wholesale). */
 int uqk_set_factor_reuse(uqk_ctx* ctx, int32_t on);
 /* Residual check before the Newton kernel in the time loop (no counterpart
- * in the reference; on by default; wave kernel shapes, uniform orders, not
- * with SolveClosureSafe).  Closure::SolveClosure stops at once when
- * ||grad L||_2 < epsilon at the cell's lambda (closure.cpp:133-193); a
- * kernel with one lane per cell evaluates that gradient for every cell,
- * finishes the cells that pass (uQ, the moments, ComputeDt, status, zero
- * iterations) and lists the others, which the Newton kernel then takes.
- * Lambda and U at the points are the Newton kernel's; the results agree
- * with the check off to rounding. */
+ * in the reference; on by default; not with SolveClosureSafe).
+ * Closure::SolveClosure stops at once when ||grad L||_2 < epsilon at the
+ * cell's lambda (closure.cpp:133-193); a check kernel evaluates that
+ * gradient for every cell -- one lane per cell for the one-wavefront
+ * kernel's shapes, one wave per cell for the workgroup kernel's (n up to
+ * 128, per-cell orders too) -- finishes the cells that pass (uQ, the
+ * moments, ComputeDt, status, zero iterations) and lists the others, which
+ * the Newton kernel then takes.  Lambda and U at the points are the Newton
+ * kernel's; the results agree with the check off to rounding. */
 int uqk_set_residual_check(uqk_ctx* ctx, int32_t on);
 int uqk_invalidate_factors(uqk_ctx* ctx);
 /* Cholesky factorisations and steps on a kept inverse of the last closure

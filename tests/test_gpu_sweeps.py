@@ -314,3 +314,48 @@ def test_residual_check_leaves_results_unchanged(precision):
     assert err.max() < 1e-12
     assert abs(on[5] - off[5]) <= 0.005 * off[5]
     assert on[6] == pytest.approx(off[6], rel=1e-12)
+
+
+@pytest.mark.parametrize("case", ["euler2d_orders_1_6", "sw2d_barrier_dambreak"])
+def test_residual_check_workgroup_kernel(case):
+    """The residual check of the workgroup kernel (one wave per cell, per-cell
+    orders too) against the same loop without it: per-cell orders 1..6 on
+    the NACA0012 O-grid (n up to 112) and the shallow-water dam break with
+    the barrier closure (n = 105); every cell converged, E/Var equal to
+    1e-10 (the projection's summation order differs), the same Newton steps
+    to 1%, and the check finishes a share of the cells."""
+    if case == "euler2d_orders_1_6":
+        b, m, u0, lam0, nt = workloads.euler2d_c4(128, 32)
+        kind, kw, steps = native.EULER2D, dict(epsilon=1e-6), 20
+        prob = native.problem(native.EULER2D_PROBLEM, flux=native.FLUX_LAX_FRIEDRICHS, cfl=0.4)
+    else:
+        b, m, u0 = workloads.sw2d_c5(64)
+        nt = None
+        kind, kw, steps = native.SHALLOW_WATER2D_BARRIER, dict(barrier=0.01, epsilon=1e-8), 10
+        prob = native.problem(native.SHALLOW_WATER2D_PROBLEM, flux=native.FLUX_REFERENCE, cfl=0.45, friction=0.01,
+                              friction_dim=2)
+        # the plain start (DS(u_0), 0, ...): the barrier closure's DS of the mean state
+        h, hu, hv = u0[:, 0, 0], u0[:, 0, 1], u0[:, 0, 2]
+        lam0 = np.zeros_like(u0)
+        lam0[:, 0] = np.stack([9.81 * h - 0.01 / h - 0.5 * ((hu / h) ** 2 + (hv / h) ** 2), hu / h, hv / h], 1)
+    out = {}
+    for check in (False, True):
+        print("case: workgroup residual check", case, check, flush=True)
+        cl = native.GpuClosure(kind, b, **kw)
+        cl.set_residual_check(check)
+        s = solver.GpuMomentSolver(cl, prob, m, nt_cell=nt)
+        s.set_state(u0, lam0)
+        s.run_device(1e3, max_steps=steps)
+        assert s.steps == steps
+        assert cl.last_kernel() == native.KERNEL_BLOCK
+        out[check] = (s.mean_var(), s.status.download(), s.iters.download(), s.newton_steps, s.t)
+    off, on = out[False], out[True]
+    assert (on[1] == 0).all() and (off[1] == 0).all()
+    finished = int((on[2] == 0).sum())
+    err = _rel_l2(on[0], off[0], m.area)
+    print("finished by the check in the last step:", finished, "of", m.n_cells, "Newton steps on/off", on[3], off[3],
+          "E/Var rel L2", err.max(), flush=True)
+    assert finished > 0
+    assert err.max() < 1e-10
+    assert abs(on[3] - off[3]) <= 0.01 * off[3] + 2
+    assert on[4] == pytest.approx(off[4], rel=1e-10)

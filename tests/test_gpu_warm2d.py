@@ -155,9 +155,15 @@ def test_warm_start_under_per_cell_orders(oracle):
     err = (np.abs(warm - lam_o) / scale).max()
     print("Newton steps plain", plain_steps, "warm", warm_steps, "oracle", int(it_o.sum()), "max rel err", err,
           flush=True)
-    assert err < 1e-8
+    # the learned start changes the Newton path: the result is the oracle's
+    # fixed point to the Newton tolerance (||grad L|| < 1e-9, checked by the
+    # oracle's gradient), not iterate for iterate
+    assert err < 1e-7
     for j in range(n_test):
         assert not warm[j, nt_test[j]:].any()
+        k = int(nt_test[j])
+        gn = np.linalg.norm(oracle.gradient(c, b.phi, b.phiw, np.ascontiguousarray(warm[j]), ut[j], nt=k))
+        assert gn < 2e-9, (j, gn)
     assert warm_steps < plain_steps
 
 

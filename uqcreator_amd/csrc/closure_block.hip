@@ -97,10 +97,15 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
     long* qshare = reinterpret_cast<long*>(rowbuf + 1024);
     const ClosureParams& c = a.c;
 
-    // guided work queue over workgroups (as in the wave kernels, per block)
+    // guided work queue over workgroups (as in the wave kernels, per block;
+    // after a residual check over its list of cells, bit-reversed)
     unsigned long long w_fail = 0, w_newton = 0, w_grad = 0;
     double w_dt = 1e10;  // min ComputeDt over this workgroup's cells (fused path)
-    const long qsize = 1L << a.qbits;
+    const long nlist = a.cell_list ? (long)*a.cell_count : 0;
+    int lbits = 0;
+    while ((1L << lbits) < nlist) ++lbits;
+    const int qb = a.cell_list ? lbits : a.qbits;
+    const long qsize = 1L << qb;
     long qpos = 0, qend = 0;
     for (;;) {
         if (qpos == qend) {
@@ -116,7 +121,12 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
         }
         const unsigned pos = (unsigned)qpos;
         ++qpos;
-        const long cell = a.qbits ? (long)(__builtin_bitreverse32(pos) >> (32 - a.qbits)) : 0;
+        const long qcell = qb ? (long)(__builtin_bitreverse32(pos) >> (32 - qb)) : 0;
+        long cell = qcell;
+        if (a.cell_list) {
+            if (qcell >= nlist) continue;
+            cell = a.cell_list[qcell];
+        }
         if (cell >= a.ncell) continue;
         const int nt = a.nt_cell ? a.nt_cell[cell] : M;
         const int n = S * nt;

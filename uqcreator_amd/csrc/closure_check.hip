@@ -109,6 +109,114 @@ __global__ void __launch_bounds__(kCheckThreads) closure_check_kernel(NewtonArgs
     }
 }
 
+// The same check for the workgroup kernel's sizes (n = S*M up to 128, any
+// Q, per-cell orders): one wave per cell, the workgroup's waves strided over
+// the cells.  Lanes over points evaluate Lambda (over the cell's nTotal), U
+// and uQ; lanes over duals (i, s), i < nTotal, sum the projection over the
+// points in order; a wave keeps the cells it leaves in LDS and appends them
+// 64 at a time.  Tables (Q x M, k-major) in LDS.
+template <int KIND>
+__global__ void __launch_bounds__(512) closure_check_wave_kernel(NewtonArgs a) {
+    constexpr int S = KindInfo<KIND>::S;
+    extern __shared__ double smem[];
+    if (a.active && *a.active == 0) return;
+    const int M = a.M, Q = a.Q, SM = S * M;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    double* tphi = smem;
+    double* tphiw = tphi + Q * M;
+    for (int t = threadIdx.x; t < Q * M; t += blockDim.x) {
+        tphi[t] = a.phi[t];
+        tphiw[t] = a.phiw[t];
+    }
+    double* lamw = tphiw + Q * M + (size_t)wave * (SM + S * Q);
+    double* Ub = lamw + SM;
+    int* pend = reinterpret_cast<int*>(tphiw + Q * M + (size_t)nw * (SM + S * Q)) + 64 * wave;
+    __syncthreads();
+    const ClosureParams& c = a.c;
+    int np = 0;  // cells pending in this wave's buffer (wave-uniform)
+    auto flush = [&]() {
+        wave_sync();
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(a.list_count, (unsigned long long)np);
+        base = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32)) << 32) |
+               (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
+        if (lane < np) a.list_out[base + lane] = pend[lane];
+        np = 0;
+        wave_sync();
+    };
+    unsigned long long w_done = 0;
+    double w_dt = 1e10;
+    const long stride = (long)gridDim.x * nw;
+    for (long cell = (long)blockIdx.x * nw + wave; cell < a.ncell; cell += stride) {
+        const int nt = a.nt_cell ? a.nt_cell[cell] : M;
+        const double* lg = a.lam + cell * SM;
+        const double* ug = a.u + cell * SM;
+        for (int t = lane; t < S * nt; t += 64) lamw[t] = lg[t];
+        wave_sync();
+        double m = 1e10;
+        const double dx = a.dt_min ? a.dx_param[cell] : 0.0;
+        for (int k = lane; k < Q; k += 64) {
+            double L[S], U[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) L[s] = 0.0;
+            for (int i = 0; i < nt; ++i) {
+                const double f = tphi[k * M + i];
+#pragma unroll
+                for (int s = 0; s < S; ++s) L[s] += lamw[i * S + s] * f;
+            }
+            closure_U<KIND>(c, L, U);
+#pragma unroll
+            for (int s = 0; s < S; ++s) Ub[s * Q + k] = U[s];
+            if (a.dt_min) m = fmin(m, point_dt<S>(a.dtp, U, dx));
+        }
+        wave_sync();
+        double sq = 0.0;
+        double acc[2] = {0.0, 0.0};  // this lane's duals e = lane, lane + 64 (n <= 128)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int e = lane + 64 * h;
+            if (e < S * nt) {
+                const int i = e / S, s = e - i * S;
+                double v = 0.0;
+                for (int k = 0; k < Q; ++k) v = fma(Ub[s * Q + k], tphiw[k * M + i], v);
+                acc[h] = v;
+                const double g = v - ug[e];
+                sq += g * g;
+            }
+        }
+        const double ng = sqrt(wave_sum(sq));
+        if (!(ng < c.eps)) {
+            if (lane == 0) pend[np] = (int)cell;
+            ++np;
+            if (np == 64) flush();
+            continue;
+        }
+        ++w_done;
+        double* uqg = a.uq_out + cell * (long)Q * S;
+        for (int t = lane; t < Q * S; t += 64) {
+            const int k = t / S, s = t - k * S;
+            uqg[t] = Ub[s * Q + k];
+        }
+        double* uo = a.u_out + cell * SM;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int e = lane + 64 * h;
+            if (e < SM) uo[e] = e < S * nt ? acc[h] : 0.0;
+        }
+        if (a.dt_min) w_dt = fmin(w_dt, wave_min(m));
+        if (lane == 0) {
+            if (a.status) a.status[cell] = UQK_STATUS_CONVERGED;
+            if (a.iters) a.iters[cell] = 0;
+        }
+        wave_sync();
+    }
+    if (np > 0) flush();
+    if (lane == 0) {
+        if (w_done) atomicAdd(a.nfail + 2, w_done);
+        if (a.dt_min && w_dt < 1e10) atomicMin(a.dt_min, (unsigned long long)__double_as_longlong(w_dt));
+    }
+}
+
 template <int KIND, int M, int Q>
 static bool check_try(int kind, int m, int q, const NewtonArgs& a, int num_cu, hipStream_t st, hipError_t* err) {
     if (kind != KIND || m != M || q != Q) return false;
@@ -126,6 +234,46 @@ bool closure_check_launch(int kind, int M, int Q, const NewtonArgs& a, int num_c
            check_try<UQK_EULER2D, 10, 25>(kind, M, Q, a, num_cu, st, err) ||
            check_try<UQK_SHALLOW_WATER2D, 10, 25>(kind, M, Q, a, num_cu, st, err) ||
            check_try<UQK_SHALLOW_WATER2D_BARRIER, 10, 25>(kind, M, Q, a, num_cu, st, err);
+}
+
+size_t closure_check_wave_lds(int S, int M, int Q, int waves) {
+    return sizeof(double) * (2 * (size_t)Q * M + (size_t)waves * ((size_t)S * M + (size_t)S * Q)) +
+           sizeof(int) * 64 * (size_t)waves;
+}
+
+template <int KIND>
+static hipError_t check_wave_go(const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st) {
+    constexpr int S = KindInfo<KIND>::S;
+    if (S * a.M > 128) return hipErrorInvalidValue;
+    int waves = 8;
+    while (waves > 1 && closure_check_wave_lds(S, a.M, a.Q, waves) > (size_t)max_lds) waves /= 2;
+    const size_t shmem = closure_check_wave_lds(S, a.M, a.Q, waves);
+    if (shmem > (size_t)max_lds) return hipErrorInvalidValue;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, closure_check_wave_kernel<KIND>, 64 * waves, shmem) !=
+            hipSuccess ||
+        occ <= 0)
+        occ = 1;
+    const long want = (a.ncell + waves - 1) / waves;
+    const long cap = (long)occ * (num_cu > 0 ? num_cu : 1);
+    const unsigned blocks = (unsigned)(want < cap ? (want > 0 ? want : 1) : cap);
+    hipLaunchKernelGGL(closure_check_wave_kernel<KIND>, dim3(blocks), dim3(64 * waves), shmem, st, a);
+    return hipGetLastError();
+}
+
+bool closure_check_wave_fits(int S, int M, int Q, int max_lds) {
+    return S * M <= 128 && closure_check_wave_lds(S, M, Q, 1) <= (size_t)max_lds;
+}
+
+hipError_t closure_check_wave_launch(int kind, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st) {
+    switch (kind) {
+        case UQK_BOUNDED_BARRIER: return check_wave_go<UQK_BOUNDED_BARRIER>(a, max_lds, num_cu, st);
+        case UQK_EULER1D: return check_wave_go<UQK_EULER1D>(a, max_lds, num_cu, st);
+        case UQK_EULER2D: return check_wave_go<UQK_EULER2D>(a, max_lds, num_cu, st);
+        case UQK_SHALLOW_WATER2D: return check_wave_go<UQK_SHALLOW_WATER2D>(a, max_lds, num_cu, st);
+        case UQK_SHALLOW_WATER2D_BARRIER: return check_wave_go<UQK_SHALLOW_WATER2D_BARRIER>(a, max_lds, num_cu, st);
+    }
+    return hipErrorInvalidValue;
 }
 
 }  // namespace uqk

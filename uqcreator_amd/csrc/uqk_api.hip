@@ -276,6 +276,17 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         a.uq_out = fuse ? uq_out : nullptr;
         a.u_out = fuse ? u_out : nullptr;
         if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev0, ctx->stream));
+        // the time loop's step: the residual check (one wave per cell) first
+        if (fuse && ctx->residual_check && !ctx->c.safe &&
+            uqk::closure_check_wave_fits(ctx->S, ctx->M, ctx->Q, (int)ctx->max_lds)) {
+            if (int rc = ensure_check_list(ctx, n)) return rc;
+            uqk::NewtonArgs ca = a;
+            ca.list_out = ctx->check_list;
+            ca.list_count = counters + 8;
+            UQK_HIP(ctx, uqk::closure_check_wave_launch(ctx->kind, ca, (int)ctx->max_lds, ctx->num_cu, ctx->stream));
+            a.cell_list = ctx->check_list;
+            a.cell_count = counters + 8;
+        }
         UQK_HIP(ctx, uqk::newton_block_launch(ctx->kind, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream, ctx->shape));
         if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev1, ctx->stream));
         ctx->last_kernel = UQK_KERNEL_BLOCK;
@@ -828,7 +839,7 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
     if (int rc = uqk::prepare_warm_start(ctx, mesh->n_cells)) return rc;
     if (ctx->precision == UQK_MIXED && ctx->factor_reuse && !nt && mesh->n_cells > 0)
         if (int rc = ensure_factor_buffers(ctx, mesh->n_cells)) return rc;
-    if (ctx->residual_check && !nt && mesh->n_cells > 0)
+    if (ctx->residual_check && mesh->n_cells > 0)
         if (int rc = ensure_check_list(ctx, mesh->n_cells)) return rc;
     UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *ctx->h_run = uqk::RunState{};

@@ -118,6 +118,10 @@ bool newton_wave_launch(int kind, int M, int Q, const NewtonArgs& a, int max_lds
 // and a.list_count receive the cells left to the Newton kernel; false when
 // the shape has no check kernel.
 bool closure_check_launch(int kind, int M, int Q, const NewtonArgs& a, int num_cu, hipStream_t st, hipError_t* err);
+// The same check for the workgroup kernel (n = S*M <= 128, per-cell orders):
+// one wave per cell (closure_check.hip).
+bool closure_check_wave_fits(int S, int M, int Q, int max_lds);
+hipError_t closure_check_wave_launch(int kind, const NewtonArgs& a, int max_lds, int num_cu, hipStream_t st);
 // Whether (kind, M, Q) has the mixed wave kernel with kept inverses.
 bool newton_wave_reuse_fits(int kind, int M, int Q, int max_lds);
 // Inverses of the factors the last mixed wave launch stored and listed
