@@ -156,7 +156,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
     // the *cell_count listed cells instead: position t is
     // cell_list[bitreverse(t)] (skipped past the count).  Newton and failure
     // counts are summed per wave (wave-uniform, scalar) and added once at the
-    // end.
+    // end.  The next cell is taken from the queue while the current one
+    // starts, and its moments, duals and flags are loaded before the current
+    // cell's results are stored, so neither latency stands between cells.
     unsigned long long w_fail = 0, w_newton = 0, w_grad = 0, w_took = 0, w_factor = 0, w_reuse = 0;
     double w_dt = 1e10;  // min ComputeDt over this wave's cells (fused path)
     const long nlist = a.cell_list ? (long)__builtin_amdgcn_readfirstlane((int)*a.cell_count) : 0;
@@ -165,33 +167,59 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
     const int qb = a.cell_list ? lbits : a.qbits;
     const long qsize = 1L << qb;
     long qpos = 0, qend = 0;
-    for (;;) {
-        if (qpos == qend) {
-            long want = (qsize - qend) / (2 * a.waves);
-            if (want < 1) want = 1;
-            unsigned long long got = 0;
-            if (lane == 0) got = atomicAdd(a.counter, (unsigned long long)want);
-            const long base = (long)(((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(got >> 32)) << 32) |
-                                     (unsigned)__builtin_amdgcn_readfirstlane((unsigned)got));
-            if (base >= qsize) break;
-            qpos = base;
-            qend = base + want < qsize ? base + want : qsize;
+    // the next cell of the queue, -1 when it is exhausted (listed cells are
+    // valid by construction)
+    auto fetch = [&]() -> long {
+        for (;;) {
+            if (qpos == qend) {
+                long want = (qsize - qend) / (2 * a.waves);
+                if (want < 1) want = 1;
+                unsigned long long got = 0;
+                if (lane == 0) got = atomicAdd(a.counter, (unsigned long long)want);
+                const long base =
+                    (long)(((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(got >> 32)) << 32) |
+                           (unsigned)__builtin_amdgcn_readfirstlane((unsigned)got));
+                if (base >= qsize) return -1;
+                qpos = base;
+                qend = base + want < qsize ? base + want : qsize;
+            }
+            const unsigned pos = (unsigned)qpos;
+            ++qpos;
+            const long qcell = qb ? (long)(__builtin_bitreverse32(pos) >> (32 - qb)) : 0;
+            if (a.cell_list) {
+                if (qcell >= nlist) continue;
+                return (long)__builtin_amdgcn_readfirstlane(a.cell_list[qcell]);
+            }
+            if (qcell < a.ncell) return qcell;
         }
-        const unsigned pos = (unsigned)qpos;
-        ++qpos;
-        const long qcell = qb ? (long)(__builtin_bitreverse32(pos) >> (32 - qb)) : 0;
-        long cell = qcell;
-        if (a.cell_list) {
-            if (qcell >= nlist) continue;
-            cell = (long)__builtin_amdgcn_readfirstlane(a.cell_list[qcell]);
+    };
+    // a cell's inputs, loaded ahead (lane t < n holds entry t of the moments,
+    // the duals and the learned start): the kept-inverse flag and the
+    // ComputeDt length are wave-uniform
+    double u_n = 0.0, l_n = 0.0, la_n = 0.0, dx_n = 0.0;
+    int kf_n = 0;
+    auto prefetch = [&](long cl) {
+        if (cl < 0) return;
+        if (lane < SM) {
+            u_n = a.u[cl * SM + lane];
+            l_n = a.lam[cl * SM + lane];
+            if (a.lam_alt) la_n = a.lam_alt[cl * SM + lane];
         }
-        if (cell >= a.ncell) continue;
-        const double* ug = a.u + cell * SM;
+        if (a.uq_out && a.dt_min) dx_n = a.dx_param[cl];
+        if constexpr (PREC == 2) kf_n = (a.pinv_mode == 1 && cl < a.pinv_slots) ? a.pinv_ok[cl] : 0;
+    };
+    long next = fetch();
+    prefetch(next);
+    while (next >= 0) {
+        const long cell = next;
+        const double la_c = la_n, dx_c = dx_n;
+        const int kf_c = kf_n;
         double* lg = a.lam + cell * SM;
-        for (int t = lane; t < SM; t += 64) {
-            uu[t] = ug[t];
-            lamA[t] = lg[t];
+        if (lane < SM) {
+            uu[lane] = u_n;
+            lamA[lane] = l_n;
         }
+        next = fetch();
         wave_sync();
         const double ue = lane < N ? uu[ei * S + el] : 0.0;
 
@@ -643,8 +671,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
         if (a.lam_alt && (c.safe || !(ng < c.eps))) {
             // learned start: Newton starts from the candidate with the smaller
             // ||grad L|| (a NaN norm fails the comparison and never wins)
-            const double* la = a.lam_alt + cell * SM;
-            for (int t = lane; t < SM; t += 64) lamnew[t] = la[t];
+            if (lane < SM) lamnew[lane] = la_c;
             double Lk0[S];
 #pragma unroll
             for (int s = 0; s < S; ++s) Lk0[s] = Lk[s];
@@ -675,7 +702,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
         if constexpr (PREC == 2) {
             if (a.pinv_mode == 1 && c.max_iter != 1 && cell < a.pinv_slots) {
                 reuse = true;
-                refactor = a.pinv_ok[cell] != 1;
+                refactor = kf_c != 1;
             }
         }
         // PREC 4 (one closure call): the factor of this cell's last Cholesky
@@ -798,6 +825,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
             if (nnew < c.eps) status = UQK_STATUS_CONVERGED;
         }
         if (status < 0) status = UQK_STATUS_MAX_ITERATIONS;
+        prefetch(next);
         // the cell's slot for the next launch: a new factor to invert (2),
         // the kept inverse still fresh (1), or none (0)
         if (PREC == 2 && reuse && lane == 0) {
@@ -821,7 +849,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                     double uk[S];
 #pragma unroll
                     for (int s = 0; s < S; ++s) uk[s] = Ub[s * QP + lane];
-                    m = point_dt<S>(a.dtp, uk, a.dx_param[cell]);
+                    m = point_dt<S>(a.dtp, uk, dx_c);
                 }
                 w_dt = fmin(w_dt, wave_min(m));
             }
