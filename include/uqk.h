@@ -245,22 +245,27 @@ double uqk_last_closure_ms(const uqk_ctx* ctx);
  * uqk_run_steps, the totals over the run's steps. */
 int uqk_last_closure_stats(const uqk_ctx* ctx, int64_t* newton_steps, int64_t* gradient_evaluations);
 /* Reuse of the factorisation in the mixed closure (no counterpart in the
- * reference; on by default for UQK_MIXED contexts; the time loop, wave
- * kernel, uniform orders).  After a cell's FP64 Cholesky of its
- * (FP32-assembled) Hessian in one step, the library computes the inverse of
- * the factored matrix (FP64, kept in FP32); the cell's Newton steps in later
- * time steps take d = P g refined once with the FP64 Hessian-vector product
- * (d += P (g - H d)) and keep it when the full step lowers ||grad L|| (the
- * FP64 accept test); otherwise, and after a contraction clearly worse than a
- * damped Newton step's (above 1.2 (1 - alpha) + 1e-3), the cell's current
- * Hessian is factored and the reference Newton step runs.  Memory per cell:
- * n (n + 1) floats and n (n + 3) / 2 doubles.  In a single closure call
+ * reference; on by default for UQK_MIXED contexts).  In the time loop, after
+ * a cell's FP64 Cholesky of its (FP32-assembled) Hessian in one step, the
+ * library keeps P = H^{-1} of the factored matrix: the wave kernel (uniform
+ * orders, n < 64) as an explicit inverse computed after the launch (FP64,
+ * kept in FP32), the workgroup kernel (n up to 128, per-cell orders too) as
+ * the factor U rounded to FP32 and tagged with the cell's order.  The cell's
+ * Newton steps in later time steps take d = P g refined once with the FP64
+ * Hessian-vector product (d += P (g - H d)) and keep it when the full step
+ * lowers ||grad L|| (the FP64 accept test); otherwise, and after a
+ * contraction clearly worse than a damped Newton step's (above
+ * 1.2 (1 - alpha) + 1e-3), the cell's current Hessian is factored and the
+ * reference Newton step runs.  Memory per cell: n (n + 1) floats and
+ * n (n + 3) / 2 doubles (wave kernel), n (n + 1) / 2 floats (workgroup
+ * kernel), n = S M.  In a single closure call
  * (wave kernel, uniform orders, damped Newton: alpha < 1, whose steps
  * contract linearly anyway) a cell's Newton steps after a factorisation
  * solve with that factor (d = (U^T U)^{-1} g) under the same accept and
  * contraction tests, with no memory beyond the call.  Results match the
- * plain mixed closure to the Newton tolerance, not bit for bit.  uqk_invalidate_factors forgets the kept
- * inverses (after replacing the duals wholesale). */
+ * plain mixed closure to the Newton tolerance, not bit for bit.
+ * uqk_invalidate_factors forgets the kept factorisations (after replacing the
+ * duals wholesale). */
 int uqk_set_factor_reuse(uqk_ctx* ctx, int32_t on);
 /* Residual check before the Newton kernel in the time loop (no counterpart
  * in the reference; on by default; not with SolveClosureSafe).

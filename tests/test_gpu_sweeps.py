@@ -316,14 +316,9 @@ def test_residual_check_leaves_results_unchanged(precision):
     assert on[6] == pytest.approx(off[6], rel=1e-12)
 
 
-@pytest.mark.parametrize("case", ["euler2d_orders_1_6", "sw2d_barrier_dambreak"])
-def test_residual_check_workgroup_kernel(case):
-    """The residual check of the workgroup kernel (one wave per cell, per-cell
-    orders too) against the same loop without it: per-cell orders 1..6 on
-    the NACA0012 O-grid (n up to 112) and the shallow-water dam break with
-    the barrier closure (n = 105); every cell converged, E/Var equal to
-    1e-10 (the projection's summation order differs), the same Newton steps
-    to 1%, and the check finishes a share of the cells."""
+def _workgroup_loop_case(case):
+    """(basis, mesh, u0, lam0, nTotal per cell, closure kind, closure kwargs,
+    steps, problem) of the workgroup kernel's time-loop tests."""
     if case == "euler2d_orders_1_6":
         b, m, u0, lam0, nt = workloads.euler2d_c4(128, 32)
         kind, kw, steps = native.EULER2D, dict(epsilon=1e-6), 20
@@ -338,6 +333,61 @@ def test_residual_check_workgroup_kernel(case):
         h, hu, hv = u0[:, 0, 0], u0[:, 0, 1], u0[:, 0, 2]
         lam0 = np.zeros_like(u0)
         lam0[:, 0] = np.stack([9.81 * h - 0.01 / h - 0.5 * ((hu / h) ** 2 + (hv / h) ** 2), hu / h, hv / h], 1)
+    return b, m, u0, lam0, nt, kind, kw, steps, prob
+
+
+@pytest.mark.parametrize("case", ["euler2d_orders_1_6", "sw2d_barrier_dambreak"])
+def test_workgroup_kept_factors(case):
+    """The mixed time loop on the workgroup kernel keeps each cell's last
+    factor (FP32, tagged with the cell's order) and takes the next steps'
+    Newton steps on it, refined once against the FP64 Hessian-vector
+    product: per-cell orders 1..6 on the NACA0012 O-grid (n up to 112) and
+    the shallow-water dam break (n = 105) end converged everywhere, with
+    E/Var within 1e-6 (rel L2) of the FP64 loop, fewer factorisations than
+    the plain mixed loop, and the replayed graph equal to the host-enqueued
+    loop bit for bit."""
+    b, m, u0, lam0, nt, kind, kw, steps, prob = _workgroup_loop_case(case)
+    out = {}
+    for prec, reuse, graph in ((native.FP64, False, True), (native.MIXED, False, True), (native.MIXED, True, True),
+                               (native.MIXED, True, False)):
+        print("case: workgroup kept factors", case, "precision", prec, "reuse", reuse, "graph", graph, flush=True)
+        cl = native.GpuClosure(kind, b, precision=prec, **kw)
+        cl.set_factor_reuse(reuse)
+        s = solver.GpuMomentSolver(cl, prob, m, nt_cell=nt)
+        s.set_state(u0, lam0)
+        s.run_device(1e3, max_steps=steps, use_graph=graph)
+        assert s.steps == steps
+        assert cl.last_kernel() == native.KERNEL_BLOCK
+        out[(prec, reuse, graph)] = (s.mean_var(), s.status.download(), s.lam.download(), s.u.download(),
+                                     s.iters.download(), s.newton_steps, cl.last_closure_factorizations())
+    ref = out[(native.FP64, False, True)]
+    plain, kept, host = out[(native.MIXED, False, True)], out[(native.MIXED, True, True)], out[(native.MIXED, True, False)]
+    for r in (ref, plain, kept):
+        assert (r[1] == 0).all()
+    err_plain = _rel_l2(plain[0], ref[0], m.area).max()
+    err_kept = _rel_l2(kept[0], ref[0], m.area).max()
+    print("E/Var rel L2 vs FP64: plain mixed", err_plain, "kept factors", err_kept, "Newton steps", ref[5], plain[5],
+          kept[5], "factorisations, steps on kept factors", plain[6], kept[6], flush=True)
+    assert err_plain < 1e-6 and err_kept < 1e-6
+    assert plain[6][1] == 0 and plain[6][0] == plain[5]
+    assert kept[6][1] > 0
+    # (the airfoil's flow settles: most steps run on kept factors; the dam
+    # break's moving front refactors more cells)
+    assert kept[6][0] < (0.5 if case == "euler2d_orders_1_6" else 0.9) * plain[6][0]
+    for x, y in zip(kept[1:5], host[1:5]):
+        np.testing.assert_array_equal(x, y)
+    assert kept[5:] == host[5:]
+
+
+@pytest.mark.parametrize("case", ["euler2d_orders_1_6", "sw2d_barrier_dambreak"])
+def test_residual_check_workgroup_kernel(case):
+    """The residual check of the workgroup kernel (one wave per cell, per-cell
+    orders too) against the same loop without it: per-cell orders 1..6 on
+    the NACA0012 O-grid (n up to 112) and the shallow-water dam break with
+    the barrier closure (n = 105); every cell converged, E/Var equal to
+    1e-10 (the projection's summation order differs), the same Newton steps
+    to 1%, and the check finishes a share of the cells."""
+    b, m, u0, lam0, nt, kind, kw, steps, prob = _workgroup_loop_case(case)
     out = {}
     for check in (False, True):
         print("case: workgroup residual check", case, check, flush=True)

@@ -17,7 +17,13 @@
 //     solves in blocks of 16 unknowns (a 16-lane sequential solve, then a
 //     workgroup-wide update of the rest);
 //   * every branch of the Newton iteration depends on workgroup-uniform
-//     values (reduced norms, pivots read from LDS).
+//     values (reduced norms, pivots read from LDS);
+//   * the time loop's mixed closure (PREC 2) keeps each cell's last factor
+//     U, its 16 x 16 diagonal blocks inverted, rounded to FP32, in a per-cell
+//     slot: while it stays good, the next steps' Newton steps solve with it
+//     and refine once against the FP64 Hessian-vector product
+//     (d = P g + P (g - H P g), P = (U^T U)^-1), with no Hessian and no
+//     factorisation.
 // The tables phi / phiw are read from global memory (L2-resident): at these
 // sizes they would take the LDS the Hessian needs.
 #include <hip/hip_runtime.h>
@@ -99,8 +105,11 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
 
     // guided work queue over workgroups (as in the wave kernels, per block;
     // after a residual check over its list of cells, bit-reversed)
-    unsigned long long w_fail = 0, w_newton = 0, w_grad = 0;
+    unsigned long long w_fail = 0, w_newton = 0, w_grad = 0, w_factor = 0, w_reuse = 0;
     double w_dt = 1e10;  // min ComputeDt over this workgroup's cells (fused path)
+    // kept factors (PREC 2): floats per cell slot, the packed U of the
+    // largest order
+    const long fslot = (long)SM * (SM + 1) / 2;
     const long nlist = a.cell_list ? (long)*a.cell_count : 0;
     int lbits = 0;
     while ((1L << lbits) < nlist) ++lbits;
@@ -137,31 +146,23 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
             lam[t] = lg[t];
             lamnew[t] = lg[t];
         }
+        // the kept factor (PREC 2) is usable when pinv_ok[cell] is this
+        // cell's nTotal (the order it was factored at), 0 when there is none
+        bool reuse = false, refactor = true, factored = false, flds = false;
+        if constexpr (PREC == 2) {
+            if (a.pinv_mode == 1 && c.max_iter != 1 && cell < a.pinv_slots) {
+                reuse = true;
+                refactor = a.pinv_ok[cell] != nt;
+            }
+        }
         __syncthreads();
 
         double Lk[S];  // Lambda at this thread's point (tid < Q) of the last gradient
         double uproj = 0.0;  // <U phiW> of this thread's dual (tid < n) at the last gradient
-        // gradient at X into gout (state-major e = l*nt + i); returns ||g||_2
-        auto gradient = [&](const double* X, double* gout) -> double {
-            // phase: gradient
-            if (tid < Q) {
-                const int k = tid;
-                double Uv[S];
-#pragma unroll
-                for (int s = 0; s < S; ++s) Lk[s] = 0.0;
-                for (int i = 0; i < nt; ++i) {
-                    const double p = phi[k * M + i];
-#pragma unroll
-                    for (int s = 0; s < S; ++s) Lk[s] += X[i * S + s] * p;
-                }
-                closure_U<KIND>(c, Lk, Uv);
-#pragma unroll
-                for (int s = 0; s < S; ++s) Ub[s * Q + k] = Uv[s];
-            }
-            __syncthreads();
-            // <U phiW>(s, i) = sum_k U_s(k) phiW(k, i) as FP64 MFMA tiles of the
-            // product (S x Q)(Q x nt): one 16-column tile per wavefront, the
-            // table loads four k-steps ahead; results land in gout, state-major
+        // <Ub phiW>: out(s, i) = sum_k Ub(s, k) phiW(k, i), state-major
+        // (e = s nt + i), as FP64 MFMA tiles of the product (S x Q)(Q x nt):
+        // one 16-column tile per wavefront, the table loads four k-steps ahead
+        auto project = [&](double* out) {
             {
                 const int r16 = lane & 15, kq4 = lane >> 4;
                 const int tnt = (nt + 15) >> 4, ks_n = (Q + 3) >> 2;
@@ -192,10 +193,32 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
                         const int sr = 4 * e + kq4;  // element e at (4e + l/16, l%16)
-                        if (sr < S && ic < nt) gout[sr * nt + ic] = pacc[e];
+                        if (sr < S && ic < nt) out[sr * nt + ic] = pacc[e];
                     }
                 }
             }
+        };
+        // gradient at X into gout (state-major e = l*nt + i); returns ||g||_2
+        auto gradient = [&](const double* X, double* gout) -> double {
+            // phase: gradient
+            if (tid < Q) {
+                const int k = tid;
+                double Uv[S];
+#pragma unroll
+                for (int s = 0; s < S; ++s) Lk[s] = 0.0;
+                // (the table reads come from L2: eight in flight at a time)
+#pragma unroll 8
+                for (int i = 0; i < nt; ++i) {
+                    const double p = phi[k * M + i];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) Lk[s] += X[i * S + s] * p;
+                }
+                closure_U<KIND>(c, Lk, Uv);
+#pragma unroll
+                for (int s = 0; s < S; ++s) Ub[s * Q + k] = Uv[s];
+            }
+            __syncthreads();
+            project(gout);
             __syncthreads();
             double sq = 0.0;
             if (tid < n) {
@@ -209,6 +232,36 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
             const double nrm = sqrt(block_allsum(sq, red));
             // phase end: gradient
             return nrm;
+        };
+        // out = H v for the FP64 Hessian at the iterate of the last gradient
+        // (Lambda in Lk of the threads over points), without forming H: per
+        // point, w(k) = DU_k (v^T phi_k), then out = <w phiW> -- the Hessian
+        // of closure.cpp:255-271 applied to v (v and out state-major).
+        // Overwrites Ub.
+        auto hessvec = [&](const double* v, double* out) {
+            if (tid < Q) {
+                const int k = tid;
+                double Ld[S], y[P];
+#pragma unroll
+                for (int s = 0; s < S; ++s) Ld[s] = 0.0;
+#pragma unroll 8
+                for (int i = 0; i < nt; ++i) {
+                    const double p = phi[k * M + i];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) Ld[s] += v[s * nt + i] * p;
+                }
+                closure_DU<KIND>(c, Lk, y);
+#pragma unroll
+                for (int l = 0; l < S; ++l) {
+                    double w = 0.0;
+#pragma unroll
+                    for (int m = 0; m < S; ++m) w += y[l >= m ? l * (l + 1) / 2 + m : m * (m + 1) / 2 + l] * Ld[m];
+                    Ub[l * Q + k] = w;
+                }
+            }
+            __syncthreads();
+            project(out);
+            __syncthreads();
         };
 
         auto hessian = [&]() {
@@ -311,6 +364,123 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
             }
             __syncthreads();
             // phase end: hessian
+        };
+
+        // U^T y = dv, then U x = y, in place (U packed upper in H, 1/U_rr in
+        // rdiag), in blocks of 16 unknowns: the 16 x 16 diagonal block is
+        // solved by lanes 0..15 of wavefront 0 (SGPR broadcasts), the rest of
+        // the vector is updated by all threads.  dv is ready in LDS (after a
+        // barrier) on entry and on return.
+        auto forward_solve = [&]() {
+            const int nb = (n + 15) / 16;
+            for (int blk = 0; blk < nb; ++blk) {
+                const int i0 = 16 * blk, i1 = i0 + 16 < n ? i0 + 16 : n;
+                if (wave == 0) {
+                    // lane t owns y_i0+t; its column of the block and the
+                    // pivots' 1/U_ii sit in registers, read ahead of the chain
+                    const int col = i0 + (lane & 15);
+                    const bool own = lane < 16 && col < i1;
+                    double y = own ? dv[col] : 0.0;
+                    double uh[16], rd[16];
+#pragma unroll
+                    for (int t = 0; t < 16; ++t) {
+                        uh[t] = (own && i0 + t < col) ? H[bpidx(i0 + t, col)] : 0.0;
+                        rd[t] = i0 + t < i1 ? rdiag[i0 + t] : 0.0;
+                    }
+#pragma unroll
+                    for (int t = 0; t < 16; ++t) {
+                        if (i0 + t >= i1) continue;  // uniform
+                        const double yi = lane_bcast(y, t) * rd[t];
+                        if (own && col > i0 + t) y -= uh[t] * yi;
+                        if (lane == t) y = yi;
+                    }
+                    if (own) dv[col] = y;
+                }
+                __syncthreads();
+                for (int cc = i1 + tid; cc < n; cc += kBlockThreads) {
+                    double acc = dv[cc];
+                    for (int r = i0; r < i1; ++r) acc -= H[bpidx(r, cc)] * dv[r];
+                    dv[cc] = acc;
+                }
+                __syncthreads();
+            }
+        };
+        auto back_solve = [&]() {
+            const int nb = (n + 15) / 16;
+            for (int blk = nb - 1; blk >= 0; --blk) {
+                const int i0 = 16 * blk, i1 = i0 + 16 < n ? i0 + 16 : n;
+                if (wave == 0) {
+                    // lane t owns x_i0+t; its row of the block and 1/U_tt sit in
+                    // registers, so a step is one multiply, broadcast and FMA
+                    const int row = i0 + (lane & 15);
+                    const bool own = lane < 16 && row < i1;
+                    double x = own ? dv[row] : 0.0;
+                    const double rd = own ? rdiag[row] : 0.0;
+                    double uh[16];
+#pragma unroll
+                    for (int t = 0; t < 16; ++t)
+                        uh[t] = (own && i0 + t < i1 && row < i0 + t) ? H[bpidx(row, i0 + t)] : 0.0;
+#pragma unroll
+                    for (int t = 15; t >= 0; --t) {
+                        if (i0 + t >= i1) continue;  // uniform
+                        const double xt = lane_bcast(x * rd, t);
+                        x = lane == t ? xt : fma(-uh[t], xt, x);
+                    }
+                    if (own) dv[row] = x;
+                }
+                __syncthreads();
+                for (int r = tid; r < i0; r += kBlockThreads) {
+                    double acc = dv[r];
+                    for (int cc = i0; cc < i1; ++cc) acc -= H[bpidx(r, cc)] * dv[cc];
+                    dv[r] = acc;
+                }
+                __syncthreads();
+            }
+        };
+        // The same two solves with a kept factor whose 16 x 16 diagonal blocks
+        // hold their inverses W_II = U_II^-1 (upper triangular): a block's
+        // unknowns are one matrix-vector product, y_I = W_II^T t_I forward and
+        // x_I = W_II t_I back, 16 independent sums instead of a chain; the
+        // block's values pass through rowbuf to the update of the rest.
+        auto forward_solve_kept = [&]() {
+            const int nb = (n + 15) / 16;
+            for (int blk = 0; blk < nb; ++blk) {
+                const int i0 = 16 * blk, i1 = i0 + 16 < n ? i0 + 16 : n;
+                const int j = i0 + tid;
+                if (j < i1) {
+                    double y = 0.0;
+                    for (int i = i0; i <= j; ++i) y = fma(H[bpidx(i, j)], dv[i], y);
+                    rowbuf[tid] = y;
+                }
+                __syncthreads();
+                if (j < i1) dv[j] = rowbuf[tid];
+                for (int cc = i1 + tid; cc < n; cc += kBlockThreads) {
+                    double acc = dv[cc];
+                    for (int r = i0; r < i1; ++r) acc -= H[bpidx(r, cc)] * rowbuf[r - i0];
+                    dv[cc] = acc;
+                }
+                __syncthreads();
+            }
+        };
+        auto back_solve_kept = [&]() {
+            const int nb = (n + 15) / 16;
+            for (int blk = nb - 1; blk >= 0; --blk) {
+                const int i0 = 16 * blk, i1 = i0 + 16 < n ? i0 + 16 : n;
+                const int i = i0 + tid;
+                if (i < i1) {
+                    double x = 0.0;
+                    for (int jj = i; jj < i1; ++jj) x = fma(H[bpidx(i, jj)], dv[jj], x);
+                    rowbuf[tid] = x;
+                }
+                __syncthreads();
+                if (i < i1) dv[i] = rowbuf[tid];
+                for (int r = tid; r < i0; r += kBlockThreads) {
+                    double acc = dv[r];
+                    for (int cc = i0; cc < i1; ++cc) acc -= H[bpidx(r, cc)] * rowbuf[cc - i0];
+                    dv[r] = acc;
+                }
+                __syncthreads();
+            }
         };
 
         // d <- H^{-1} g (U^T U = H, upper, packed); false if not SPD
@@ -491,60 +661,9 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
             __syncthreads();
             // phase end: factor
             // phase: backsub
-            // U^T y = g (unless eliminated with the factor) then U x = y, in
-            // blocks of 16 unknowns: the 16 x 16 diagonal block is solved by
-            // lanes 0..15 of wavefront 0 (SGPR broadcasts), the rest of the
-            // vector is updated by all threads.
-            const int nb = (n + 15) / 16;
-            for (int blk = 0; blk < (aug ? 0 : nb); ++blk) {
-                const int i0 = 16 * blk, i1 = i0 + 16 < n ? i0 + 16 : n;
-                if (wave == 0) {
-                    const int col = i0 + (lane & 15);
-                    double y = (lane < 16 && col < i1) ? dv[col] : 0.0;
-                    for (int i = i0; i < i1; ++i) {
-                        const double yi = lane_bcast(y, i - i0) * rdiag[i];
-                        if (lane < 16 && col > i && col < i1) y -= H[bpidx(i, col)] * yi;
-                        if (lane == i - i0) y = yi;
-                    }
-                    if (lane < 16 && col < i1) dv[col] = y;
-                }
-                __syncthreads();
-                for (int cc = i1 + tid; cc < n; cc += kBlockThreads) {
-                    double acc = dv[cc];
-                    for (int r = i0; r < i1; ++r) acc -= H[bpidx(r, cc)] * dv[r];
-                    dv[cc] = acc;
-                }
-                __syncthreads();
-            }
-            for (int blk = nb - 1; blk >= 0; --blk) {
-                const int i0 = 16 * blk, i1 = i0 + 16 < n ? i0 + 16 : n;
-                if (wave == 0) {
-                    // lane t owns x_i0+t; its row of the block and 1/U_tt sit in
-                    // registers, so a step is one multiply, broadcast and FMA
-                    const int row = i0 + (lane & 15);
-                    const bool own = lane < 16 && row < i1;
-                    double x = own ? dv[row] : 0.0;
-                    const double rd = own ? rdiag[row] : 0.0;
-                    double uh[16];
-#pragma unroll
-                    for (int t = 0; t < 16; ++t)
-                        uh[t] = (own && i0 + t < i1 && row < i0 + t) ? H[bpidx(row, i0 + t)] : 0.0;
-#pragma unroll
-                    for (int t = 15; t >= 0; --t) {
-                        if (i0 + t >= i1) continue;  // uniform
-                        const double xt = lane_bcast(x * rd, t);
-                        x = lane == t ? xt : fma(-uh[t], xt, x);
-                    }
-                    if (own) dv[row] = x;
-                }
-                __syncthreads();
-                for (int r = tid; r < i0; r += kBlockThreads) {
-                    double acc = dv[r];
-                    for (int cc = i0; cc < i1; ++cc) acc -= H[bpidx(r, cc)] * dv[cc];
-                    dv[r] = acc;
-                }
-                __syncthreads();
-            }
+            // U^T y = g (unless eliminated with the factor) then U x = y
+            if (!aug) forward_solve();
+            back_solve();
             // phase end: backsub
             return true;
         };
@@ -556,18 +675,83 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
         int status = -1;
         int iters = 0;
         if (!c.safe && ng < c.eps) status = UQK_STATUS_CONVERGED;
+        // trial point lambda - step alpha d (d in dv): its gradient into gnew
+        auto trial = [&](double step) -> double {
+            if (tid < n) {
+                const int l = tid / nt, i = tid - l * nt;
+                lamnew[i * S + l] = lam[i * S + l] - step * c.alpha * dv[tid];
+            }
+            __syncthreads();
+            ++ngrad;
+            return gradient(lamnew, gnew);
+        };
         for (int it = 0; status < 0 && it < c.max_iter; ++it) {
             if (it > 0) {
                 double* t = g;
                 g = gnew;
                 gnew = t;
             }
-            hessian();
             ++iters;
+            if constexpr (PREC == 2) {
+                if (reuse && !refactor) {
+                    // a step on the kept factor: d = P g + P (g - H P g) with
+                    // P = (U^T U)^-1 of the kept FP32 factor and the FP64
+                    // Hessian at lambda; kept when the full step lowers
+                    // ||grad L|| (the FP64 accept test), else lambda stays and
+                    // the next iteration factors its own Hessian
+                    if (!flds) {
+                        const float* fs = a.pinv + cell * fslot;
+                        const int hp = n * (n + 1) / 2;
+                        for (int t = tid; t < hp; t += kBlockThreads) H[t] = (double)fs[t];
+                        flds = true;
+                    }
+                    for (int t = tid; t < n; t += kBlockThreads) dv[t] = g[t];
+                    __syncthreads();
+                    forward_solve_kept();
+                    back_solve_kept();
+                    const double d1 = tid < n ? dv[tid] : 0.0;
+                    hessvec(dv, gnew);  // (the trial gradient rewrites gnew)
+                    if (tid < n) dv[tid] = g[tid] - gnew[tid];
+                    __syncthreads();
+                    forward_solve_kept();
+                    back_solve_kept();
+                    if (tid < n) dv[tid] += d1;
+                    __syncthreads();
+                    double Lk0[S];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) Lk0[s] = Lk[s];
+                    const double nnew = trial(1.0);
+                    if (nnew < ng) {
+                        ++w_reuse;
+                        double* t = lam;
+                        lam = lamnew;
+                        lamnew = t;
+                        const double ratio = nnew / ng;
+                        ng = nnew;
+                        // contraction clearly worse than a (damped) Newton
+                        // step's: factor afresh at the next Newton step
+                        if (!(ratio <= 1.2 * (1.0 - c.alpha) + 1e-3)) refactor = true;
+                        if (nnew < c.eps) status = UQK_STATUS_CONVERGED;
+                    } else {
+#pragma unroll
+                        for (int s = 0; s < S; ++s) Lk[s] = Lk0[s];
+                        // g stays: the swap at the next iteration's start
+                        // brings it back
+                        double* t = g;
+                        g = gnew;
+                        gnew = t;
+                        refactor = true;
+                    }
+                    continue;
+                }
+            }
+            hessian();
             if (!cholesky_solve(g)) {
                 status = UQK_STATUS_NOT_SPD;
                 break;
             }
+            ++w_factor;
+            factored = reuse;
             if (c.max_iter == 1) {  // one-shot IPM (closure.cpp:151-154)
                 if (tid < n) {
                     const int l = tid / nt, i = tid - l * nt;
@@ -577,15 +761,6 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                 status = UQK_STATUS_CONVERGED;
                 break;
             }
-            auto trial = [&](double step) -> double {
-                if (tid < n) {
-                    const int l = tid / nt, i = tid - l * nt;
-                    lamnew[i * S + l] = lam[i * S + l] - step * c.alpha * dv[tid];
-                }
-                __syncthreads();
-                ++ngrad;
-                return gradient(lamnew, gnew);
-            };
             double step = 1.0;
             double nnew = trial(step);
             int halvings = 0;
@@ -623,6 +798,39 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
             if (nnew < c.eps) status = UQK_STATUS_CONVERGED;
         }
         if (status < 0) status = UQK_STATUS_MAX_ITERATIONS;
+        if constexpr (PREC == 2) {
+            // the slot for the next steps: this call's last factor (in H and
+            // rdiag) with its 16 x 16 diagonal blocks inverted, rounded to
+            // FP32, tagged with the cell's order; no factor after a step on
+            // the kept one did poorly
+            if (reuse) {
+                const bool keep = factored && status != UQK_STATUS_NOT_SPD;
+                if (keep) {
+                    // W_II = U_II^-1 in place, by the 16 threads 16 I + i of
+                    // block I (one wavefront), a column at a time (W U = I):
+                    // W_jj = 1/U_jj, W_ij = -(sum_{i <= k < j} W_ik U_kj) / U_jj
+                    const int b0 = 16 * (tid >> 4), il = tid & 15, ii = b0 + il;
+                    for (int j = 0; j < 16; ++j) {
+                        const int jc = b0 + j;
+                        const bool col = jc < n;
+                        double t = 0.0;
+                        if (col && il < j)
+                            for (int k = il; k < j; ++k) t = fma(H[bpidx(ii, b0 + k)], H[bpidx(b0 + k, jc)], t);
+                        wave_sync();
+                        if (col && il < j)
+                            H[bpidx(ii, jc)] = -t * rdiag[jc];
+                        else if (col && il == j)
+                            H[bpidx(jc, jc)] = rdiag[jc];
+                        wave_sync();
+                    }
+                    __syncthreads();
+                    float* fs = a.pinv + cell * fslot;
+                    const int hp = n * (n + 1) / 2;
+                    for (int t = tid; t < hp; t += kBlockThreads) fs[t] = (float)H[t];
+                }
+                if (tid == 0 && (keep || refactor)) a.pinv_ok[cell] = keep ? nt : 0;
+            }
+        }
         for (int t = tid; t < SM; t += kBlockThreads) lg[t] = lam[t];
         if (a.uq_out) {
             // the step's next two sweeps from the last gradient, which is at
@@ -660,6 +868,10 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
         if (w_fail) atomicAdd(a.nfail, w_fail);
         if (w_newton) atomicAdd(a.nfail + 1, w_newton);
         if (w_grad) atomicAdd(a.nfail + 2, w_grad);
+        if (a.nfactor) {
+            if (w_factor) atomicAdd(a.nfactor, w_factor);
+            if (w_reuse) atomicAdd(a.nfactor + 1, w_reuse);
+        }
     }
     // positive doubles order as their bit patterns; one atomic per wavefront
     if (a.dt_min && lane == 0 && w_dt < 1e10) atomicMin(a.dt_min, (unsigned long long)__double_as_longlong(w_dt));
@@ -670,15 +882,20 @@ size_t newton_block_lds(int S, int M, int Q) {
     return sizeof(double) * ((size_t)SM * (SM + 1) / 2 + 6 * (size_t)SM + (size_t)S * Q + (size_t)Q * P + SM + 8 + 256 + 1024 + 2);
 }
 
+// the instantiation for a launch: FP64, mixed, or mixed with kept factors
+inline int block_prec(int prec, int pinv_mode) { return prec == UQK_FP64 ? 0 : (pinv_mode == 1 ? 2 : 1); }
+
 template <int KIND>
-static int block_occupancy(int M, int Q, int prec) {
+static int block_occupancy(int M, int Q, int prec, int pinv_mode = 0) {
     constexpr int S = KindInfo<KIND>::S;
     const size_t shmem = newton_block_lds(S, M, Q);
     int occ = 0;
-    const hipError_t e =
-        prec == UQK_FP64
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, newton_block_kernel<KIND, 0>, kBlockThreads, shmem)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, newton_block_kernel<KIND, 1>, kBlockThreads, shmem);
+    hipError_t e;
+    switch (block_prec(prec, pinv_mode)) {
+        case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, newton_block_kernel<KIND, 0>, kBlockThreads, shmem); break;
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, newton_block_kernel<KIND, 1>, kBlockThreads, shmem); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, newton_block_kernel<KIND, 2>, kBlockThreads, shmem);
+    }
     return e == hipSuccess && occ > 0 ? occ : 1;
 }
 
@@ -687,7 +904,7 @@ static hipError_t block_go(const NewtonArgs& args, int max_lds, int num_cu, hipS
     constexpr int S = KindInfo<KIND>::S;
     const size_t shmem = newton_block_lds(S, args.M, args.Q);
     if (shmem > (size_t)max_lds || S * args.M > kBlockMaxN || args.Q > kBlockThreads) return hipErrorInvalidValue;
-    const int occ = block_occupancy<KIND>(args.M, args.Q, args.prec);
+    const int occ = block_occupancy<KIND>(args.M, args.Q, args.prec, args.pinv_mode);
     // persistent workgroups per CU: the resident maximum, or the tuned count
     const int per_cu = shape.blocks_per_cu > 0 && shape.blocks_per_cu < occ ? shape.blocks_per_cu : occ;
     long blocks = args.ncell < (long)per_cu * num_cu ? args.ncell : (long)per_cu * num_cu;
@@ -695,10 +912,11 @@ static hipError_t block_go(const NewtonArgs& args, int max_lds, int num_cu, hipS
     NewtonArgs a = args;
     // the queue counts workgroups here: one cell per workgroup at a time
     set_queue(a, blocks);
-    if (a.prec == UQK_FP64)
-        hipLaunchKernelGGL((newton_block_kernel<KIND, 0>), dim3((unsigned)blocks), dim3(kBlockThreads), shmem, st, a);
-    else
-        hipLaunchKernelGGL((newton_block_kernel<KIND, 1>), dim3((unsigned)blocks), dim3(kBlockThreads), shmem, st, a);
+    switch (block_prec(a.prec, a.pinv_mode)) {
+        case 0: hipLaunchKernelGGL((newton_block_kernel<KIND, 0>), dim3((unsigned)blocks), dim3(kBlockThreads), shmem, st, a); break;
+        case 1: hipLaunchKernelGGL((newton_block_kernel<KIND, 1>), dim3((unsigned)blocks), dim3(kBlockThreads), shmem, st, a); break;
+        default: hipLaunchKernelGGL((newton_block_kernel<KIND, 2>), dim3((unsigned)blocks), dim3(kBlockThreads), shmem, st, a);
+    }
     return hipGetLastError();
 }
 

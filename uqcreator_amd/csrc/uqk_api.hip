@@ -105,12 +105,32 @@ int check_mesh(uqk_ctx* ctx, const uqk_mesh* m) {
 // Kept inverse factors of the mixed wave kernel across time steps: grows
 // the per-cell slots (flags cleared) to n cells.  Allocates only outside
 // graph capture.
-int ensure_factor_buffers(uqk_ctx* ctx, long n_cells) {
+// Kept factorisations of the time loop's mixed closure, by the kernel the
+// launch takes: 1 the wave kernel's kept inverses, 2 the workgroup kernel's
+// kept FP32 factors, 0 none.
+int factor_mode(const uqk_ctx* ctx, bool per_cell_orders) {
+    if (ctx->precision != UQK_MIXED || !ctx->factor_reuse || ctx->c.max_iter == 1) return 0;
+    int wm = 0, wo = 0;
+    if ((ctx->kernel == UQK_KERNEL_AUTO || ctx->kernel == UQK_KERNEL_WAVE) && !per_cell_orders &&
+        uqk::newton_wave_limits(ctx->kind, ctx->M, ctx->Q, ctx->precision, (int)ctx->max_lds, 1, &wm, &wo))
+        return uqk::newton_wave_reuse_fits(ctx->kind, ctx->M, ctx->Q, (int)ctx->max_lds) ? 1 : 0;
+    if (ctx->kernel == UQK_KERNEL_WAVE) return 0;
+    const bool block_ok = uqk::newton_block_fits(ctx->S, ctx->M, ctx->Q, (int)ctx->max_lds);
+    if (block_ok && (ctx->kernel == UQK_KERNEL_BLOCK || (ctx->kernel == UQK_KERNEL_AUTO && ctx->S * ctx->M >= 64)))
+        return 2;
+    return 0;
+}
+
+// Per-cell buffers of factor mode `mode` for n_cells cells, outside graph
+// capture only; a change of mode starts with no kept factor.
+int ensure_factor_buffers(uqk_ctx* ctx, long n_cells, int mode) {
     const long n = (long)ctx->S * ctx->M;
-    const long nn = n * (n + 1);  // n x n FP32, row stride n + 1, per cell
-    const long nu = n * (n + 1) / 2 + n;  // packed U and 1/U_rr, FP64, per cell
-    if (ctx->pinv_cell_cap >= n_cells) return 0;
-    if (ctx->capturing) return fail(ctx, -1, "kept inverses: buffers must exist before graph capture");
+    // mode 1: n x n FP32 inverse (row stride n + 1) and the FP64 factor
+    // (packed U, 1/U_rr); mode 2: the FP32 factor (packed U)
+    const long nn = mode == 1 ? n * (n + 1) : n * (n + 1) / 2;
+    const long nu = n * (n + 1) / 2 + n;
+    if (ctx->pinv_cell_cap >= n_cells && ctx->factor_mode == mode) return 0;
+    if (ctx->capturing) return fail(ctx, -1, "kept factors: buffers must exist before graph capture");
     hipFree(ctx->pinv_cell);
     hipFree(ctx->ufac_cell);
     hipFree(ctx->pinv_ok);
@@ -120,12 +140,16 @@ int ensure_factor_buffers(uqk_ctx* ctx, long n_cells) {
     ctx->pinv_ok = nullptr;
     ctx->pinv_list = nullptr;
     ctx->pinv_cell_cap = 0;
+    ctx->factor_mode = 0;
     UQK_HIP(ctx, hipMalloc(&ctx->pinv_cell, sizeof(float) * (size_t)nn * n_cells));
-    UQK_HIP(ctx, hipMalloc(&ctx->ufac_cell, sizeof(double) * (size_t)nu * n_cells));
+    if (mode == 1) {
+        UQK_HIP(ctx, hipMalloc(&ctx->ufac_cell, sizeof(double) * (size_t)nu * n_cells));
+        UQK_HIP(ctx, hipMalloc(&ctx->pinv_list, sizeof(int) * (size_t)n_cells));
+    }
     UQK_HIP(ctx, hipMalloc(&ctx->pinv_ok, sizeof(int) * (size_t)n_cells));
-    UQK_HIP(ctx, hipMalloc(&ctx->pinv_list, sizeof(int) * (size_t)n_cells));
     UQK_HIP(ctx, hipMemsetAsync(ctx->pinv_ok, 0, sizeof(int) * (size_t)n_cells, ctx->stream));
     ctx->pinv_cell_cap = n_cells;
+    ctx->factor_mode = mode;
     return 0;
 }
 
@@ -188,14 +212,13 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     a.active = active;
     a.took = counters + 4;
     a.nfactor = counters + 5;
-    // mixed precision: kept inverses (per cell in the time loop, per wave for
-    // a single closure call); the buffers exist before a graph is captured
-    // (the time loop only: each cell's inverse is computed after the launch
-    // that factored, for the next steps)
-    if (ctx->precision == UQK_MIXED && ctx->factor_reuse && !nt && ctx->c.max_iter != 1 && factors_per_cell &&
-        (ctx->kernel == UQK_KERNEL_AUTO || ctx->kernel == UQK_KERNEL_WAVE) &&
-        uqk::newton_wave_reuse_fits(ctx->kind, ctx->M, ctx->Q, (int)ctx->max_lds)) {
-        if (int rc = ensure_factor_buffers(ctx, n)) return rc;
+    // mixed precision: kept factorisations (per cell in the time loop: the
+    // wave kernel's inverses, computed after the launch that factored, or the
+    // workgroup kernel's FP32 factors; per wave for a single closure call);
+    // the buffers exist before a graph is captured
+    const int fmode = factors_per_cell ? factor_mode(ctx, nt != nullptr) : 0;
+    if (fmode) {
+        if (int rc = ensure_factor_buffers(ctx, n, fmode)) return rc;
         a.pinv_mode = 1;
         a.pinv = ctx->pinv_cell;
         a.ufac = ctx->ufac_cell;
@@ -272,6 +295,7 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
     const bool block_ok = uqk::newton_block_fits(ctx->S, ctx->M, ctx->Q, (int)ctx->max_lds);
     if (ctx->kernel == UQK_KERNEL_BLOCK || (ctx->kernel == UQK_KERNEL_AUTO && ctx->S * ctx->M >= 64 && block_ok)) {
         if (!block_ok) return fail(ctx, -3, "closure too large for the workgroup kernel");
+        if (a.pinv_mode == 1 && ctx->factor_mode != 2) a.pinv_mode = 0;  // (the slots hold inverses)
         const bool fuse = uq_out && u_out && ctx->c.max_iter != 1;
         a.uq_out = fuse ? uq_out : nullptr;
         a.u_out = fuse ? u_out : nullptr;
@@ -837,8 +861,9 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
     // the warm start's and the kept inverses' buffers must exist before the
     // step is captured
     if (int rc = uqk::prepare_warm_start(ctx, mesh->n_cells)) return rc;
-    if (ctx->precision == UQK_MIXED && ctx->factor_reuse && !nt && mesh->n_cells > 0)
-        if (int rc = ensure_factor_buffers(ctx, mesh->n_cells)) return rc;
+    if (const int fmode = factor_mode(ctx, nt != nullptr))
+        if (mesh->n_cells > 0)
+            if (int rc = ensure_factor_buffers(ctx, mesh->n_cells, fmode)) return rc;
     if (ctx->residual_check && mesh->n_cells > 0)
         if (int rc = ensure_check_list(ctx, mesh->n_cells)) return rc;
     UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -66,6 +66,9 @@ struct uqk_ctx {
     double* ufac_cell = nullptr;
     int* pinv_ok = nullptr;
     int* pinv_list = nullptr;  // cells with a new factor, per launch
+    // what the buffers hold: 1 the wave kernel's kept inverses (with ufac
+    // and pinv_list), 2 the workgroup kernel's kept FP32 factors; 0 none
+    int factor_mode = 0;
     // residual check before the wave kernel in the time loop (on by
     // default): the cells it leaves to the Newton kernel
     bool residual_check = true;
