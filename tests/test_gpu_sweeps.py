@@ -379,6 +379,43 @@ def test_workgroup_kept_factors(case):
     assert kept[5:] == host[5:]
 
 
+def test_workgroup_warm_start_in_check():
+    """The workgroup path's residual check chooses the learned start for the
+    cells that fail at their lambda (the wave kernel's rule: the candidate
+    with the smaller ||grad L||): on the shallow-water dam break (n = 105)
+    the FP64 loop with the network attached ends converged everywhere, with
+    E/Var within 1e-7 of the plain loop, both with the check on (the choice
+    fused into it) and off (a choice pass over every cell)."""
+    b, m, u0, lam0, nt, kind, kw, steps, prob = _workgroup_loop_case("sw2d_barrier_dambreak")
+    cl = native.GpuClosure(kind, b, **kw)
+    s = solver.GpuMomentSolver(cl, prob, m)
+    s.set_state(u0, lam0)
+    s.run_device(1e3, max_steps=steps)
+    ref = s.mean_var()
+    net = native.WarmStartNet(cl, hidden=64, seed=4)
+    for i, lr in enumerate((1e-3, 3e-4)):
+        net.train(s.u, s.lam, batch=1024, steps=300, lr=lr, seed=i + 1)
+    params, norm = net.get_params()
+    for check in (True, False):
+        print("case: workgroup warm start, residual check", check, flush=True)
+        cw = native.GpuClosure(kind, b, **kw)
+        cw.set_residual_check(check)
+        nw = native.WarmStartNet(cw, hidden=64, seed=4)
+        nw.set_params(params, norm)
+        nw.attach()
+        sw = solver.GpuMomentSolver(cw, prob, m)
+        sw.set_state(u0, lam0)
+        sw.run_device(1e3, max_steps=steps)
+        assert sw.steps == steps
+        assert cw.last_kernel() == native.KERNEL_BLOCK
+        assert (sw.status.download() == 0).all()
+        err = _rel_l2(sw.mean_var(), ref, m.area).max()
+        print("E/Var rel L2 vs the plain loop", err, "Newton steps", sw.newton_steps, "plain", s.newton_steps,
+              flush=True)
+        assert err < 1e-7
+        nw.detach()
+
+
 @pytest.mark.parametrize("case", ["euler2d_orders_1_6", "sw2d_barrier_dambreak"])
 def test_residual_check_workgroup_kernel(case):
     """The residual check of the workgroup kernel (one wave per cell, per-cell

@@ -114,7 +114,11 @@ __global__ void __launch_bounds__(kCheckThreads) closure_check_kernel(NewtonArgs
 // the cells.  Lanes over points evaluate Lambda (over the cell's nTotal), U
 // and uQ; lanes over duals (i, s), i < nTotal, sum the projection over the
 // points in order; a wave keeps the cells it leaves in LDS and appends them
-// 64 at a time.  Tables (Q x M, k-major) in LDS.
+// 64 at a time.  Tables (Q x M, k-major) in LDS.  With a learned start
+// (a.lam_alt), a cell that fails at its lambda is also evaluated at the
+// network's candidate, which replaces lambda when its ||grad L|| is smaller
+// (a NaN norm never wins), as the wave kernel chooses; a candidate that
+// passes finishes the cell.
 template <int KIND>
 __global__ void __launch_bounds__(512) closure_check_wave_kernel(NewtonArgs a) {
     constexpr int S = KindInfo<KIND>::S;
@@ -144,47 +148,66 @@ __global__ void __launch_bounds__(512) closure_check_wave_kernel(NewtonArgs a) {
         np = 0;
         wave_sync();
     };
-    unsigned long long w_done = 0;
+    unsigned long long w_done = 0, w_took = 0, w_alt = 0;
     double w_dt = 1e10;
     const long stride = (long)gridDim.x * nw;
     for (long cell = (long)blockIdx.x * nw + wave; cell < a.ncell; cell += stride) {
         const int nt = a.nt_cell ? a.nt_cell[cell] : M;
-        const double* lg = a.lam + cell * SM;
+        double* lg = a.lam + cell * SM;
         const double* ug = a.u + cell * SM;
-        for (int t = lane; t < S * nt; t += 64) lamw[t] = lg[t];
-        wave_sync();
         double m = 1e10;
         const double dx = a.dt_min ? a.dx_param[cell] : 0.0;
-        for (int k = lane; k < Q; k += 64) {
-            double L[S], U[S];
-#pragma unroll
-            for (int s = 0; s < S; ++s) L[s] = 0.0;
-            for (int i = 0; i < nt; ++i) {
-                const double f = tphi[k * M + i];
-#pragma unroll
-                for (int s = 0; s < S; ++s) L[s] += lamw[i * S + s] * f;
-            }
-            closure_U<KIND>(c, L, U);
-#pragma unroll
-            for (int s = 0; s < S; ++s) Ub[s * Q + k] = U[s];
-            if (a.dt_min) m = fmin(m, point_dt<S>(a.dtp, U, dx));
-        }
-        wave_sync();
-        double sq = 0.0;
         double acc[2] = {0.0, 0.0};  // this lane's duals e = lane, lane + 64 (n <= 128)
+        // ||grad L|| at the duals X (the cell's nTotal of them); leaves U at
+        // the points in Ub, this lane's projections in acc, min ComputeDt in m
+        auto eval = [&](const double* X) -> double {
+            for (int t = lane; t < S * nt; t += 64) lamw[t] = X[t];
+            wave_sync();
+            m = 1e10;
+            for (int k = lane; k < Q; k += 64) {
+                double L[S], U[S];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int e = lane + 64 * h;
-            if (e < S * nt) {
-                const int i = e / S, s = e - i * S;
-                double v = 0.0;
-                for (int k = 0; k < Q; ++k) v = fma(Ub[s * Q + k], tphiw[k * M + i], v);
-                acc[h] = v;
-                const double g = v - ug[e];
-                sq += g * g;
+                for (int s = 0; s < S; ++s) L[s] = 0.0;
+                for (int i = 0; i < nt; ++i) {
+                    const double f = tphi[k * M + i];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) L[s] += lamw[i * S + s] * f;
+                }
+                closure_U<KIND>(c, L, U);
+#pragma unroll
+                for (int s = 0; s < S; ++s) Ub[s * Q + k] = U[s];
+                if (a.dt_min) m = fmin(m, point_dt<S>(a.dtp, U, dx));
+            }
+            wave_sync();
+            double sq = 0.0;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int e = lane + 64 * h;
+                acc[h] = 0.0;
+                if (e < S * nt) {
+                    const int i = e / S, s = e - i * S;
+                    double v = 0.0;
+                    for (int k = 0; k < Q; ++k) v = fma(Ub[s * Q + k], tphiw[k * M + i], v);
+                    acc[h] = v;
+                    const double g = v - ug[e];
+                    sq += g * g;
+                }
+            }
+            const double nrm = sqrt(wave_sum(sq));
+            wave_sync();
+            return nrm;
+        };
+        double ng = eval(lg);
+        if (!(ng < c.eps) && a.lam_alt) {
+            const double* la = a.lam_alt + cell * SM;
+            const double nalt = eval(la);
+            ++w_alt;
+            if (nalt < ng) {
+                for (int t = lane; t < SM; t += 64) lg[t] = la[t];
+                ng = nalt;
+                ++w_took;
             }
         }
-        const double ng = sqrt(wave_sum(sq));
         if (!(ng < c.eps)) {
             if (lane == 0) pend[np] = (int)cell;
             ++np;
@@ -212,7 +235,10 @@ __global__ void __launch_bounds__(512) closure_check_wave_kernel(NewtonArgs a) {
     }
     if (np > 0) flush();
     if (lane == 0) {
-        if (w_done) atomicAdd(a.nfail + 2, w_done);
+        // gradients: the finished cells' and every candidate's (the listed
+        // cells' first gradient counts in the Newton kernel)
+        if (w_done + w_alt) atomicAdd(a.nfail + 2, w_done + w_alt);
+        if (w_took && a.took) atomicAdd(a.took, w_took);
         if (a.dt_min && w_dt < 1e10) atomicMin(a.dt_min, (unsigned long long)__double_as_longlong(w_dt));
     }
 }

@@ -290,8 +290,6 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         if (ctx->kernel == UQK_KERNEL_WAVE) return fail(ctx, -3, "no specialised closure kernel for this shape");
         a.lam_alt = nullptr;
     }
-    if (lam_alt)
-        if (int rc = uqk::enqueue_warm_select(ctx, n, u, lam, lam_alt, active, nt)) return rc;
     const bool block_ok = uqk::newton_block_fits(ctx->S, ctx->M, ctx->Q, (int)ctx->max_lds);
     if (ctx->kernel == UQK_KERNEL_BLOCK || (ctx->kernel == UQK_KERNEL_AUTO && ctx->S * ctx->M >= 64 && block_ok)) {
         if (!block_ok) return fail(ctx, -3, "closure too large for the workgroup kernel");
@@ -299,12 +297,18 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         const bool fuse = uq_out && u_out && ctx->c.max_iter != 1;
         a.uq_out = fuse ? uq_out : nullptr;
         a.u_out = fuse ? u_out : nullptr;
+        const bool check = fuse && ctx->residual_check && !ctx->c.safe &&
+                           uqk::closure_check_wave_fits(ctx->S, ctx->M, ctx->Q, (int)ctx->max_lds);
+        // learned start: the residual check chooses it for the cells that
+        // fail at lambda; without the check a pass over every cell does
+        if (lam_alt && !check)
+            if (int rc = uqk::enqueue_warm_select(ctx, n, u, lam, lam_alt, active, nt)) return rc;
         if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev0, ctx->stream));
         // the time loop's step: the residual check (one wave per cell) first
-        if (fuse && ctx->residual_check && !ctx->c.safe &&
-            uqk::closure_check_wave_fits(ctx->S, ctx->M, ctx->Q, (int)ctx->max_lds)) {
+        if (check) {
             if (int rc = ensure_check_list(ctx, n)) return rc;
             uqk::NewtonArgs ca = a;
+            ca.lam_alt = lam_alt;
             ca.list_out = ctx->check_list;
             ca.list_count = counters + 8;
             UQK_HIP(ctx, uqk::closure_check_wave_launch(ctx->kind, ca, (int)ctx->max_lds, ctx->num_cu, ctx->stream));
@@ -319,6 +323,8 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         if (fused_dt) *fused_dt = want_dt;
         return 0;
     }
+    if (lam_alt)
+        if (int rc = uqk::enqueue_warm_select(ctx, n, u, lam, lam_alt, active, nt)) return rc;
     ctx->last_kernel = UQK_KERNEL_GENERIC;
     a.uq_out = nullptr;
     a.u_out = nullptr;
