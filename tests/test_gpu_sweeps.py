@@ -379,6 +379,40 @@ def test_workgroup_kept_factors(case):
     assert kept[5:] == host[5:]
 
 
+def test_workgroup_kept_factors_follow_orders():
+    """Kept factors are tagged with the order they were factored at: after
+    every cell's order changes mid-run (per-cell orders 1..6 on a small
+    NACA0012 O-grid, the workgroup kernel's mixed loop), no stale factor is
+    used -- the run ends bit for bit as one whose kept factors were dropped
+    at the change."""
+    b, m, u0, lam0, nt = workloads.euler2d_c4(64, 16)
+    prob = native.problem(native.EULER2D_PROBLEM, flux=native.FLUX_LAX_FRIEDRICHS, cfl=0.4)
+    levels = np.asarray(b.ntotal_per_level, np.int32)
+    lev = np.searchsorted(levels, nt)
+    nt2 = levels[(lev + 1) % len(levels)].astype(np.int32)
+    assert (nt2 != nt).all()
+    out = []
+    for drop in (False, True):
+        print("case: kept factors across an order change, dropped", drop, flush=True)
+        cl = native.GpuClosure(native.EULER2D, b, epsilon=1e-6, precision=native.MIXED)
+        s = solver.GpuMomentSolver(cl, prob, m, nt_cell=nt)
+        s.set_state(u0, lam0)
+        s.run_device(1e3, max_steps=6)
+        assert cl.last_closure_factorizations()[1] > 0
+        s.nt.upload(nt2)
+        if drop:
+            cl.invalidate_factors()
+        s.run_device(1e3, max_steps=3)
+        assert cl.last_kernel() == native.KERNEL_BLOCK
+        out.append((s.lam.download(), s.u.download(), s.status.download(), s.iters.download(),
+                    cl.last_closure_factorizations()))
+    kept, dropped = out
+    for x, y in zip(kept[:4], dropped[:4]):
+        np.testing.assert_array_equal(x, y)
+    assert kept[4] == dropped[4]
+    print("after the change: factorisations, steps on kept factors", kept[4], flush=True)
+
+
 def test_workgroup_warm_start_in_check():
     """The workgroup path's residual check chooses the learned start for the
     cells that fail at their lambda (the wave kernel's rule: the candidate
