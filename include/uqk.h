@@ -250,7 +250,9 @@ int uqk_last_closure_stats(const uqk_ctx* ctx, int64_t* newton_steps, int64_t* g
  * library keeps P = H^{-1} of the factored matrix: the wave kernel (uniform
  * orders, n < 64) as an explicit inverse computed after the launch (FP64,
  * kept in FP32), the workgroup kernel (n up to 128, per-cell orders too) as
- * the factor U rounded to FP32 and tagged with the cell's order.  The cell's
+ * the factor U rounded to FP32 and tagged with the cell's order, replaced
+ * by U^{-1} (P = U^{-1} U^{-T}) the first time a later step uses it.  The
+ * cell's
  * Newton steps in later time steps take d = P g refined once with the FP64
  * Hessian-vector product (d += P (g - H d)) and keep it when the full step
  * lowers ||grad L|| (the FP64 accept test); otherwise, and after a

@@ -169,8 +169,7 @@ def _legs(make_closure, prob, m, u, lam0, nt, steps, params, closure_from_lam0, 
             # (the workgroup kernel's mixed time loop is its kept-factor instantiation, <KIND, 2>)
             r["roofline_closure"] = roofline.closure_roofline(
                 kernels["closure"] % (2 if block and prec == native.MIXED else prec), cms, newton / steps,
-                grads / steps, fm, prec == native.MIXED, factors / steps, lagged / steps,
-                reuse="kept_factor" if block else "inverse")
+                grads / steps, fm, prec == native.MIXED, factors / steps, lagged / steps)
         print("case:", leg, "loop", {k: r[k] for k in ("value", "ms_per_step", "newton_steps_per_cell_step",
                                                        "mean_var_rel_l2_vs_fp64")}, file=sys.stderr, flush=True)
         if net is not None:

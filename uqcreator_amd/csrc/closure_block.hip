@@ -18,12 +18,13 @@
 //     workgroup-wide update of the rest);
 //   * every branch of the Newton iteration depends on workgroup-uniform
 //     values (reduced norms, pivots read from LDS);
-//   * the time loop's mixed closure (PREC 2) keeps each cell's last factor
-//     U, its 16 x 16 diagonal blocks inverted, rounded to FP32, in a per-cell
-//     slot: while it stays good, the next steps' Newton steps solve with it
-//     and refine once against the FP64 Hessian-vector product
-//     (d = P g + P (g - H P g), P = (U^T U)^-1), with no Hessian and no
-//     factorisation.
+//   * the time loop's mixed closure (PREC 2) keeps W = U^-1 of each cell's
+//     last factor (inverted in place by 16 x 16 blocks, the blocks above the
+//     diagonal on FP64 MFMA), rounded to FP32, in a per-cell slot: while it
+//     stays good, the next steps' Newton steps take d = P g + P (g - H P g),
+//     P = W W^T, against the FP64 Hessian-vector product -- four triangular
+//     matrix-vector products, with no Hessian, factorisation or triangular
+//     solve.
 // The tables phi / phiw are read from global memory (L2-resident): at these
 // sizes they would take the LDS the Hessian needs.
 #include <hip/hip_runtime.h>
@@ -42,6 +43,9 @@ constexpr int kBlockMaxN = 128;
 // register tile slots per wavefront: 8 x 8 tiles of 16 x 16 cover n <= 128,
 // 36 of them on or above the diagonal, dealt to 4 wavefronts
 constexpr int kSlots = 9;
+// pinv_ok of a kept factor (workgroup kernel): the cell's nTotal, plus this
+// bit once the slot holds the factor's inverse
+constexpr int kKeptInverted = 0x100;
 
 __device__ __forceinline__ int bpidx(int r, int c) { return c * (c + 1) / 2 + r; }  // r <= c
 
@@ -149,10 +153,12 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
         // the kept factor (PREC 2) is usable when pinv_ok[cell] is this
         // cell's nTotal (the order it was factored at), 0 when there is none
         bool reuse = false, refactor = true, factored = false, flds = false;
+        int kept_tag = 0;
         if constexpr (PREC == 2) {
             if (a.pinv_mode == 1 && c.max_iter != 1 && cell < a.pinv_slots) {
                 reuse = true;
-                refactor = a.pinv_ok[cell] != nt;
+                kept_tag = a.pinv_ok[cell];
+                refactor = (kept_tag & 0xff) != nt;
             }
         }
         __syncthreads();
@@ -437,49 +443,121 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                 __syncthreads();
             }
         };
-        // The same two solves with a kept factor whose 16 x 16 diagonal blocks
-        // hold their inverses W_II = U_II^-1 (upper triangular): a block's
-        // unknowns are one matrix-vector product, y_I = W_II^T t_I forward and
-        // x_I = W_II t_I back, 16 independent sums instead of a chain; the
-        // block's values pass through rowbuf to the update of the rest.
-        auto forward_solve_kept = [&]() {
-            const int nb = (n + 15) / 16;
-            for (int blk = 0; blk < nb; ++blk) {
-                const int i0 = 16 * blk, i1 = i0 + 16 < n ? i0 + 16 : n;
-                const int j = i0 + tid;
-                if (j < i1) {
-                    double y = 0.0;
-                    for (int i = i0; i <= j; ++i) y = fma(H[bpidx(i, j)], dv[i], y);
-                    rowbuf[tid] = y;
+        // The same two solves with the kept inverse W = U^-1 (packed upper
+        // in H): y = W^T v, then x = W y, each a matrix-vector product,
+        // thread c (r) summing column c (row r) of W in four partial sums;
+        // dv in and out.
+        auto solve_kept = [&]() {
+            double y = 0.0;
+            if (tid < n) {
+                const int c = tid;
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+                int r = 0;
+                for (; r + 3 <= c; r += 4) {
+                    s0 = fma(H[bpidx(r, c)], dv[r], s0);
+                    s1 = fma(H[bpidx(r + 1, c)], dv[r + 1], s1);
+                    s2 = fma(H[bpidx(r + 2, c)], dv[r + 2], s2);
+                    s3 = fma(H[bpidx(r + 3, c)], dv[r + 3], s3);
                 }
-                __syncthreads();
-                if (j < i1) dv[j] = rowbuf[tid];
-                for (int cc = i1 + tid; cc < n; cc += kBlockThreads) {
-                    double acc = dv[cc];
-                    for (int r = i0; r < i1; ++r) acc -= H[bpidx(r, cc)] * rowbuf[r - i0];
-                    dv[cc] = acc;
-                }
-                __syncthreads();
+                for (; r <= c; ++r) s0 = fma(H[bpidx(r, c)], dv[r], s0);
+                y = (s0 + s1) + (s2 + s3);
             }
+            __syncthreads();
+            if (tid < n) dv[tid] = y;
+            __syncthreads();
+            double x = 0.0;
+            if (tid < n) {
+                const int r = tid;
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+                int c = r;
+                for (; c + 3 < n; c += 4) {
+                    s0 = fma(H[bpidx(r, c)], dv[c], s0);
+                    s1 = fma(H[bpidx(r, c + 1)], dv[c + 1], s1);
+                    s2 = fma(H[bpidx(r, c + 2)], dv[c + 2], s2);
+                    s3 = fma(H[bpidx(r, c + 3)], dv[c + 3], s3);
+                }
+                for (; c < n; ++c) s0 = fma(H[bpidx(r, c)], dv[c], s0);
+                x = (s0 + s1) + (s2 + s3);
+            }
+            __syncthreads();
+            if (tid < n) dv[tid] = x;
+            __syncthreads();
         };
-        auto back_solve_kept = [&]() {
-            const int nb = (n + 15) / 16;
-            for (int blk = nb - 1; blk >= 0; --blk) {
-                const int i0 = 16 * blk, i1 = i0 + 16 < n ? i0 + 16 : n;
-                const int i = i0 + tid;
-                if (i < i1) {
-                    double x = 0.0;
-                    for (int jj = i; jj < i1; ++jj) x = fma(H[bpidx(i, jj)], dv[jj], x);
-                    rowbuf[tid] = x;
+
+        // W = U^-1 in place over the factor in H (1/U_rr in rdiag).
+        auto invert_factor = [&]() {
+            // W_II = U_II^-1 in place, by the 16 threads 16 I + i of
+            // block I (one wavefront), a column at a time (W U = I):
+            // W_jj = 1/U_jj, W_ij = -(sum_{i <= k < j} W_ik U_kj) / U_jj
+            const int b0 = 16 * (tid >> 4), il = tid & 15, ii = b0 + il;
+            for (int j = 0; j < 16; ++j) {
+                const int jc = b0 + j;
+                const bool col = jc < n;
+                double t = 0.0;
+                if (col && il < j)
+                    for (int k = il; k < j; ++k) t = fma(H[bpidx(ii, b0 + k)], H[bpidx(b0 + k, jc)], t);
+                wave_sync();
+                if (col && il < j)
+                    H[bpidx(ii, jc)] = -t * rdiag[jc];
+                else if (col && il == j)
+                    H[bpidx(jc, jc)] = rdiag[jc];
+                wave_sync();
+            }
+            __syncthreads();
+            // the blocks above the diagonal, a block column J at a
+            // time (W U = I): W_IJ = -T_I W_JJ, T_I = sum_{I<=K<J}
+            // W_IK U_KJ, as 16 x 16 x 4 FP64 MFMA products (wavefront
+            // w takes the rows I = w, w + 4).  W_IK (K < J) are final
+            // and U_KJ still in place while every T_I is summed; after
+            // a barrier T_I goes through the wavefront's part of rowbuf
+            // into the A layout and W_IJ replaces U_IJ.
+            {
+                const int nbk = (n + 15) / 16, r16 = lane & 15, kq4 = lane >> 4;
+                double* scr = rowbuf + 256 * wave;
+                for (int J = 1; J < nbk; ++J) {
+                    const int gcol = 16 * J + r16;
+                    f64x4 tp[2];
+#pragma unroll
+                    for (int p = 0; p < 2; ++p) {
+                        const int I = wave + 4 * p;
+                        f64x4 t = {0.0, 0.0, 0.0, 0.0};
+                        if (I < J) {
+                            const int gi = 16 * I + r16;
+                            for (int K = I; K < J; ++K)
+#pragma unroll
+                                for (int ks = 0; ks < 4; ++ks) {
+                                    const int kk = 16 * K + 4 * ks + kq4;
+                                    const double av = gi <= kk ? H[bpidx(gi, kk)] : 0.0;
+                                    const double bv = gcol < n ? H[bpidx(kk, gcol)] : 0.0;
+                                    t = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, t, 0, 0, 0);
+                                }
+                        }
+                        tp[p] = t;
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int p = 0; p < 2; ++p) {
+                        const int I = wave + 4 * p;
+                        if (I >= J) continue;
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) scr[(4 * e + kq4) * 16 + r16] = tp[p][e];
+                        wave_sync();
+                        f64x4 w = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                        for (int ks = 0; ks < 4; ++ks) {
+                            const int k = 4 * ks + kq4;
+                            const double av = scr[r16 * 16 + k];
+                            const double bv = (k <= r16 && gcol < n) ? H[bpidx(16 * J + k, gcol)] : 0.0;
+                            w = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, w, 0, 0, 0);
+                        }
+                        if (gcol < n) {
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) H[bpidx(16 * I + 4 * e + kq4, gcol)] = -w[e];
+                        }
+                        wave_sync();
+                    }
+                    __syncthreads();
                 }
-                __syncthreads();
-                if (i < i1) dv[i] = rowbuf[tid];
-                for (int r = tid; r < i0; r += kBlockThreads) {
-                    double acc = dv[r];
-                    for (int cc = i0; cc < i1; ++cc) acc -= H[bpidx(r, cc)] * rowbuf[cc - i0];
-                    dv[r] = acc;
-                }
-                __syncthreads();
             }
         };
 
@@ -700,21 +778,28 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                     // ||grad L|| (the FP64 accept test), else lambda stays and
                     // the next iteration factors its own Hessian
                     if (!flds) {
-                        const float* fs = a.pinv + cell * fslot;
+                        // the slot holds U, or W once a step has used it
+                        float* fs = a.pinv + cell * fslot;
                         const int hp = n * (n + 1) / 2;
                         for (int t = tid; t < hp; t += kBlockThreads) H[t] = (double)fs[t];
+                        if (!(kept_tag & kKeptInverted)) {
+                            __syncthreads();
+                            for (int t = tid; t < n; t += kBlockThreads) rdiag[t] = 1.0 / H[bpidx(t, t)];
+                            __syncthreads();
+                            invert_factor();
+                            for (int t = tid; t < hp; t += kBlockThreads) fs[t] = (float)H[t];
+                            if (tid == 0) a.pinv_ok[cell] = nt | kKeptInverted;
+                        }
                         flds = true;
                     }
                     for (int t = tid; t < n; t += kBlockThreads) dv[t] = g[t];
                     __syncthreads();
-                    forward_solve_kept();
-                    back_solve_kept();
+                    solve_kept();
                     const double d1 = tid < n ? dv[tid] : 0.0;
                     hessvec(dv, gnew);  // (the trial gradient rewrites gnew)
                     if (tid < n) dv[tid] = g[tid] - gnew[tid];
                     __syncthreads();
-                    forward_solve_kept();
-                    back_solve_kept();
+                    solve_kept();
                     if (tid < n) dv[tid] += d1;
                     __syncthreads();
                     double Lk0[S];
@@ -799,31 +884,14 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
         }
         if (status < 0) status = UQK_STATUS_MAX_ITERATIONS;
         if constexpr (PREC == 2) {
-            // the slot for the next steps: this call's last factor (in H and
-            // rdiag) with its 16 x 16 diagonal blocks inverted, rounded to
-            // FP32, tagged with the cell's order; no factor after a step on
-            // the kept one did poorly
+            // the slot for the next steps: this call's last factor U (in H),
+            // rounded to FP32 and tagged with the cell's order; the first
+            // step that uses it inverts it (W = U^-1, P = W W^T) and keeps W
+            // in its place.  No factor after a step on the kept one did
+            // poorly.
             if (reuse) {
                 const bool keep = factored && status != UQK_STATUS_NOT_SPD;
                 if (keep) {
-                    // W_II = U_II^-1 in place, by the 16 threads 16 I + i of
-                    // block I (one wavefront), a column at a time (W U = I):
-                    // W_jj = 1/U_jj, W_ij = -(sum_{i <= k < j} W_ik U_kj) / U_jj
-                    const int b0 = 16 * (tid >> 4), il = tid & 15, ii = b0 + il;
-                    for (int j = 0; j < 16; ++j) {
-                        const int jc = b0 + j;
-                        const bool col = jc < n;
-                        double t = 0.0;
-                        if (col && il < j)
-                            for (int k = il; k < j; ++k) t = fma(H[bpidx(ii, b0 + k)], H[bpidx(b0 + k, jc)], t);
-                        wave_sync();
-                        if (col && il < j)
-                            H[bpidx(ii, jc)] = -t * rdiag[jc];
-                        else if (col && il == j)
-                            H[bpidx(jc, jc)] = rdiag[jc];
-                        wave_sync();
-                    }
-                    __syncthreads();
                     float* fs = a.pinv + cell * fslot;
                     const int hp = n * (n + 1) / 2;
                     for (int t = tid; t < hp; t += kBlockThreads) fs[t] = (float)H[t];

@@ -43,10 +43,9 @@ def closure_roofline(kernel, kernel_ms, newton, grads, fm, precision_mixed, fact
     time; the peak is time-weighted over the units the flops run on (Hessian
     on FP64 or FP32 MFMA, the rest FP64).  factors / lagged: factored Newton
     steps and steps on a kept factorisation (mixed closure; by default every
-    Newton step is factored).  reuse: "inverse" (the wave kernel's time loop
-    keeps an inverse per factorisation, F_inv + F_lag), "kept_factor" (the
-    workgroup kernel's time loop keeps the factor: its two solve pairs per
-    step cost the two applications of F_lag, no inverse) or "factor" (a
+    Newton step is factored).  reuse: "inverse" (the time loop keeps an
+    inverse per factorisation -- the wave kernel's H^-1, the workgroup
+    kernel's U^-1 formed at its first use -- F_inv + F_lag) or "factor" (a
     single call solves with its last factor, F_tri)."""
     if factors is None:
         factors = newton
@@ -55,8 +54,6 @@ def closure_roofline(kernel, kernel_ms, newton, grads, fm, precision_mixed, fact
     if lagged:
         if reuse == "inverse":
             f_fp64 += factors * fm["F_inv"] + lagged * fm["F_lag"]
-        elif reuse == "kept_factor":
-            f_fp64 += lagged * fm["F_lag"]
         else:
             f_fp64 += lagged * fm["F_tri"]
     flops = f_hess + f_fp64
