@@ -252,17 +252,16 @@ int uqk_last_closure_stats(const uqk_ctx* ctx, int64_t* newton_steps, int64_t* g
  * kept in FP32), the workgroup kernel (n up to 128, per-cell orders too) as
  * the factor U rounded to FP32 and tagged with the cell's order, replaced
  * by U^{-1} (P = U^{-1} U^{-T}) the first time a later step uses it.  The
- * cell's
- * Newton steps in later time steps take d = P g refined once with the FP64
- * Hessian-vector product (d += P (g - H d)) and keep it when the full step
- * lowers ||grad L|| (the FP64 accept test); otherwise, and after a
- * contraction clearly worse than a damped Newton step's (above
+ * cell's Newton steps in later time steps take d = P g refined once with
+ * the FP64 Hessian-vector product (d += P (g - H d)) and keep it when the
+ * full step lowers ||grad L|| (the FP64 accept test); otherwise, and after
+ * a contraction clearly worse than a damped Newton step's (above
  * 1.2 (1 - alpha) + 1e-3), the cell's current Hessian is factored and the
  * reference Newton step runs.  Memory per cell: n (n + 1) floats and
  * n (n + 3) / 2 doubles (wave kernel), n (n + 1) / 2 floats (workgroup
- * kernel), n = S M.  In a single closure call
- * (wave kernel, uniform orders, damped Newton: alpha < 1, whose steps
- * contract linearly anyway) a cell's Newton steps after a factorisation
+ * kernel), n = S M.  In a single closure call (wave kernel, uniform
+ * orders, damped Newton: alpha < 1, whose steps contract linearly anyway)
+ * a cell's Newton steps after a factorisation
  * solve with that factor (d = (U^T U)^{-1} g) under the same accept and
  * contraction tests, with no memory beyond the call.  Results match the
  * plain mixed closure to the Newton tolerance, not bit for bit.
