@@ -330,7 +330,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000, help="timed time steps of the 2D Euler run")
     ap.add_argument("--warmup", type=int, default=20, help="untimed time steps first (then back to the initial state)")
     ap.add_argument("--legs", default="fp64,mixed,mixed_warm_start")
-    ap.add_argument("--oracle-steps", type=int, default=10,
+    ap.add_argument("--oracle-steps", type=int, default=200,
                     help="steps of the CPU oracle run: the CPU baseline and the check of every leg")
     ap.add_argument("--c3-train-snapshots", type=int, default=8)
     ap.add_argument("--c3-train-steps", type=int, default=19200)
@@ -438,7 +438,8 @@ def main():
         cpu_s, mv_o, res_o = c3.oracle_run(oracle_py, ks, nth)
         out["cpu_baseline"] = {
             "value": c3.n * ks / cpu_s, "unit": "cell-steps/s", "cores": nth, "kind": "port",
-            "sample": "the same 65,536-cell run, first %d time steps (Newton from (DS(u0), 0, ...) in the first), "
+            "sample": "the same 65,536-cell run, first %d of its time steps (the first with the Newton from "
+                      "(DS(u0), 0, ...)), "
                       "OpenMP dynamic(10) over cells, %.1f s" % (ks, cpu_s)}
         out["oracle_check"] = {"steps": ks, "oracle_newton_steps": res_o["newton_iters"]}
         for name in legs:
