@@ -13,7 +13,8 @@
 // wave kernel's order), U(Lambda_k) -> uQ (written for every cell; the
 // Newton kernel rewrites the cells it takes), ComputeDt there, and the
 // projection sum_k U(k) phiW(k, i) accumulated in k order.  The list grows
-// by one atomic per wave (ballot and prefix count).
+// by one atomic per wave (ballot and prefix count); the gradients of the
+// listed cells are stored for the Newton kernel, which starts from them.
 #include <hip/hip_runtime.h>
 
 #include "uqk_device.h"
@@ -86,6 +87,11 @@ __global__ void __launch_bounds__(kCheckThreads) closure_check_kernel(NewtonArgs
         for (int t = 0; t < N; ++t) uo[t] = acc[t];
         if (a.status) a.status[cs] = UQK_STATUS_CONVERGED;
         if (a.iters) a.iters[cs] = 0;
+    } else if (valid && a.grad_at_lam) {
+        // the gradient the Newton kernel starts from
+        double* gg = a.grad_at_lam + cs * N;
+#pragma unroll
+        for (int t = 0; t < N; ++t) gg[t] = acc[t] - ug[t];
     }
     // the others to the list: one atomic per wave
     const unsigned long long left = __ballot(valid && !conv);

@@ -194,9 +194,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
         }
     };
     // a cell's inputs, loaded ahead (lane t < n holds entry t of the moments,
-    // the duals and the learned start): the kept-inverse flag and the
-    // ComputeDt length are wave-uniform
-    double u_n = 0.0, l_n = 0.0, la_n = 0.0, dx_n = 0.0;
+    // the duals and the learned start; lane e = s M + i the residual check's
+    // gradient entry e): the kept-inverse flag and the ComputeDt length are
+    // wave-uniform
+    double u_n = 0.0, l_n = 0.0, la_n = 0.0, dx_n = 0.0, g_n = 0.0;
     int kf_n = 0;
     auto prefetch = [&](long cl) {
         if (cl < 0) return;
@@ -205,6 +206,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
             l_n = a.lam[cl * SM + lane];
             if (a.lam_alt) la_n = a.lam_alt[cl * SM + lane];
         }
+        if (a.grad_at_lam && lane < N) g_n = a.grad_at_lam[cl * SM + ei * S + el];
         if (a.uq_out && a.dt_min) dx_n = a.dx_param[cl];
         if constexpr (PREC == 2) kf_n = (a.pinv_mode == 1 && cl < a.pinv_slots) ? a.pinv_ok[cl] : 0;
     };
@@ -212,7 +214,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
     prefetch(next);
     while (next >= 0) {
         const long cell = next;
-        const double la_c = la_n, dx_c = dx_n;
+        const double la_c = la_n, dx_c = dx_n, g_c = g_n;
         const int kf_c = kf_n;
         double* lg = a.lam + cell * SM;
         if (lane < SM) {
@@ -252,10 +254,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
             wave_sync();
             return acc;
         };
-        // gradient at X (LDS); returns ||g||, leaves g_e in gout, Lambda in Lk
-        auto gradient = [&](const double* X, double& gout) -> double {
-            // phase: gradient
-            double L[S], Uv[S];
+        // Lambda at this lane's point for the duals X (LDS)
+        auto point_lambda = [&](const double* X, double* L) {
 #pragma unroll
             for (int s = 0; s < S; ++s) L[s] = 0.0;
 #pragma unroll
@@ -264,6 +264,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
 #pragma unroll
                 for (int s = 0; s < S; ++s) L[s] += X[i * S + s] * f;
             }
+        };
+        // gradient at X (LDS); returns ||g||, leaves g_e in gout, Lambda in Lk
+        auto gradient = [&](const double* X, double& gout) -> double {
+            // phase: gradient
+            double L[S], Uv[S];
+            point_lambda(X, L);
             closure_U<KIND>(c, L, Uv);
             if (lane < Q) {
 #pragma unroll
@@ -665,7 +671,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
         double* lamnew = lamB;
         double g = 0.0, gnew = 0.0, d = 0.0;
         int ngrad = 1;
-        double ng = gradient(lam, g);
+        // Ub and uproj hold U and <U phiW> at lam, the outputs' source
+        bool ub_lam = true;
+        double ng;
+        if (a.grad_at_lam) {
+            // the residual check's gradient at lambda (its projection summed in
+            // point order); Lambda at this lane's point for the first Hessian
+            point_lambda(lam, Lk);
+            g = lane < N ? g_c : 0.0;
+            ng = sqrt(wave_sum(g * g));
+            ub_lam = false;
+        } else {
+            ng = gradient(lam, g);
+        }
         int status = -1;
         int iters = 0;
         if (a.lam_alt && (c.safe || !(ng < c.eps))) {
@@ -687,7 +705,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                 g = galt;
                 ng = nalt;
                 ++w_took;
+                ub_lam = true;
             } else {
+                ub_lam = false;
                 g = g0;
 #pragma unroll
                 for (int s = 0; s < S; ++s) Lk[s] = Lk0[s];
@@ -754,6 +774,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                 }
                 if (c.max_iter == 1) {  // one-shot IPM (closure.cpp:151-154)
                     if (lane < N) lam[ei * S + el] -= c.alpha * d;
+                    ub_lam = false;
                     wave_sync();
                     status = UQK_STATUS_CONVERGED;
                     break;
@@ -769,6 +790,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                 wave_sync();
                 ++ngrad;
                 nnew = gradient(lamnew, gnew);
+                ub_lam = false;
                 if (halvings > 0) {
                     if (nnew < c.eps) {
                         status = UQK_STATUS_CONVERGED;
@@ -792,6 +814,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                     double* t = lam;
                     lam = lamnew;
                     lamnew = t;
+                    ub_lam = true;
                     const double ratio = nnew / ng;
                     ng = nnew;
                     // contraction clearly worse than a (damped) Newton step's:
@@ -811,12 +834,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                     double* t = lam;
                     lam = lamnew;
                     lamnew = t;
+                    ub_lam = true;
                 }
                 break;
             }
             double* t = lam;
             lam = lamnew;
             lamnew = t;
+            ub_lam = true;
             ng = nnew;
             if (!isfinite(nnew)) {
                 status = UQK_STATUS_NONFINITE;
@@ -825,6 +850,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
             if (nnew < c.eps) status = UQK_STATUS_CONVERGED;
         }
         if (status < 0) status = UQK_STATUS_MAX_ITERATIONS;
+        if (a.uq_out && !ub_lam) {
+            // the outputs come from U at the returned lambda: evaluated again
+            // where the last gradient was another point's (a rejected step, a
+            // failed cell, or the check's gradient alone)
+            double gt;
+            ++ngrad;
+            gradient(lam, gt);
+        }
         prefetch(next);
         // the cell's slot for the next launch: a new factor to invert (2),
         // the kept inverse still fresh (1), or none (0)

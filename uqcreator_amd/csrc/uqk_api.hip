@@ -153,17 +153,38 @@ int ensure_factor_buffers(uqk_ctx* ctx, long n_cells, int mode) {
     return 0;
 }
 
-// The list of cells the residual check leaves to the Newton kernel: grown to
-// n cells, outside graph capture only.
-int ensure_check_list(uqk_ctx* ctx, long n_cells) {
-    if (ctx->check_cap >= n_cells) return 0;
-    if (ctx->capturing) return fail(ctx, -1, "residual check: the cell list must exist before graph capture");
-    hipFree(ctx->check_list);
-    ctx->check_list = nullptr;
-    ctx->check_cap = 0;
-    UQK_HIP(ctx, hipMalloc(&ctx->check_list, sizeof(int) * (size_t)(n_cells > 0 ? n_cells : 1)));
-    ctx->check_cap = n_cells;
+// The list of cells the residual check leaves to the Newton kernel and, for
+// the wave kernel (grad), their gradients at lambda: grown to n cells,
+// outside graph capture only.
+int ensure_check_list(uqk_ctx* ctx, long n_cells, bool grad) {
+    const size_t cells = (size_t)(n_cells > 0 ? n_cells : 1);
+    if (ctx->check_cap < n_cells) {
+        if (ctx->capturing) return fail(ctx, -1, "residual check: the cell list must exist before graph capture");
+        hipFree(ctx->check_list);
+        ctx->check_list = nullptr;
+        ctx->check_cap = 0;
+        UQK_HIP(ctx, hipMalloc(&ctx->check_list, sizeof(int) * cells));
+        ctx->check_cap = n_cells;
+    }
+    if (grad && ctx->check_grad_cap < n_cells) {
+        if (ctx->capturing) return fail(ctx, -1, "residual check: the gradients must exist before graph capture");
+        hipFree(ctx->check_grad);
+        ctx->check_grad = nullptr;
+        ctx->check_grad_cap = 0;
+        UQK_HIP(ctx, hipMalloc(&ctx->check_grad, sizeof(double) * (size_t)ctx->S * ctx->M * cells));
+        ctx->check_grad_cap = n_cells;
+    }
     return 0;
+}
+
+// Whether the time loop's step runs the residual check before the wave
+// kernel: a wave kernel shape, no per-cell orders (the wave kernel takes
+// none), not SolveClosureSafe (which always takes a step).
+bool wave_check(const uqk_ctx* ctx, bool per_cell_orders) {
+    int wm = 0, wo = 0;
+    return (ctx->kernel == UQK_KERNEL_AUTO || ctx->kernel == UQK_KERNEL_WAVE) && ctx->residual_check &&
+           !ctx->c.safe && !per_cell_orders &&
+           uqk::newton_wave_limits(ctx->kind, ctx->M, ctx->Q, ctx->precision, (int)ctx->max_lds, 1, &wm, &wo);
 }
 
 // Launch the Newton closure kernel; leaves the failed-cell count in counters[1]
@@ -254,19 +275,19 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         a.lam_alt = lam_alt;
         if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev0, ctx->stream));
         // the time loop's step: the residual check finishes the cells whose
-        // lambda already passes the convergence test (not SolveClosureSafe,
-        // which always takes a step) and lists the rest for the wave kernel
-        int wm = 0, wo = 0;
-        if (fuse && ctx->residual_check && !ctx->c.safe &&
-            uqk::newton_wave_limits(ctx->kind, ctx->M, ctx->Q, ctx->precision, (int)ctx->max_lds, 1, &wm, &wo)) {
-            if (int rc = ensure_check_list(ctx, n)) return rc;
+        // lambda already passes the convergence test and lists the rest, with
+        // their gradients, for the wave kernel
+        if (fuse && wave_check(ctx, nt != nullptr)) {
+            if (int rc = ensure_check_list(ctx, n, true)) return rc;
             uqk::NewtonArgs ca = a;
             ca.list_out = ctx->check_list;
             ca.list_count = counters + 8;
+            ca.grad_at_lam = ctx->check_grad;
             if (uqk::closure_check_launch(ctx->kind, ctx->M, ctx->Q, ca, ctx->num_cu, ctx->stream, &e)) {
                 UQK_HIP(ctx, e);
                 a.cell_list = ctx->check_list;
                 a.cell_count = counters + 8;
+                a.grad_at_lam = ctx->check_grad;
             }
         }
         if (uqk::newton_wave_launch(ctx->kind, ctx->M, ctx->Q, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream, &e,
@@ -289,6 +310,9 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         }
         if (ctx->kernel == UQK_KERNEL_WAVE) return fail(ctx, -3, "no specialised closure kernel for this shape");
         a.lam_alt = nullptr;
+        a.cell_list = nullptr;
+        a.cell_count = nullptr;
+        a.grad_at_lam = nullptr;
     }
     const bool block_ok = uqk::newton_block_fits(ctx->S, ctx->M, ctx->Q, (int)ctx->max_lds);
     if (ctx->kernel == UQK_KERNEL_BLOCK || (ctx->kernel == UQK_KERNEL_AUTO && ctx->S * ctx->M >= 64 && block_ok)) {
@@ -306,7 +330,7 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev0, ctx->stream));
         // the time loop's step: the residual check (one wave per cell) first
         if (check) {
-            if (int rc = ensure_check_list(ctx, n)) return rc;
+            if (int rc = ensure_check_list(ctx, n, false)) return rc;
             uqk::NewtonArgs ca = a;
             ca.lam_alt = lam_alt;
             ca.list_out = ctx->check_list;
@@ -582,6 +606,7 @@ int uqk_destroy(uqk_ctx* ctx) {
     hipFree(ctx->pinv_ok);
     hipFree(ctx->pinv_list);
     hipFree(ctx->check_list);
+    hipFree(ctx->check_grad);
     if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
     hipFree(ctx->d_run);
     if (ctx->h_run) hipHostFree(ctx->h_run);
@@ -871,7 +896,7 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
         if (mesh->n_cells > 0)
             if (int rc = ensure_factor_buffers(ctx, mesh->n_cells, fmode)) return rc;
     if (ctx->residual_check && mesh->n_cells > 0)
-        if (int rc = ensure_check_list(ctx, mesh->n_cells)) return rc;
+        if (int rc = ensure_check_list(ctx, mesh->n_cells, wave_check(ctx, nt != nullptr))) return rc;
     UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *ctx->h_run = uqk::RunState{};
     ctx->h_run->t = t0;
