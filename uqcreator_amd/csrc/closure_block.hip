@@ -204,21 +204,26 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                 }
             }
         };
+        // Lambda at this thread's point (tid < Q) for the duals X, into Lk
+        auto point_lambda = [&](const double* X) {
+            const int k = tid;
+#pragma unroll
+            for (int s = 0; s < S; ++s) Lk[s] = 0.0;
+            // (the table reads come from L2: eight in flight at a time)
+#pragma unroll 8
+            for (int i = 0; i < nt; ++i) {
+                const double p = phi[k * M + i];
+#pragma unroll
+                for (int s = 0; s < S; ++s) Lk[s] += X[i * S + s] * p;
+            }
+        };
         // gradient at X into gout (state-major e = l*nt + i); returns ||g||_2
         auto gradient = [&](const double* X, double* gout) -> double {
             // phase: gradient
             if (tid < Q) {
                 const int k = tid;
                 double Uv[S];
-#pragma unroll
-                for (int s = 0; s < S; ++s) Lk[s] = 0.0;
-                // (the table reads come from L2: eight in flight at a time)
-#pragma unroll 8
-                for (int i = 0; i < nt; ++i) {
-                    const double p = phi[k * M + i];
-#pragma unroll
-                    for (int s = 0; s < S; ++s) Lk[s] += X[i * S + s] * p;
-                }
+                point_lambda(X);
                 closure_U<KIND>(c, Lk, Uv);
 #pragma unroll
                 for (int s = 0; s < S; ++s) Ub[s * Q + k] = Uv[s];
@@ -749,7 +754,25 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
         double* g = gA;
         double* gnew = gB;
         int ngrad = 1;
-        double ng = gradient(lam, g);
+        // Ub and uproj hold U and <U phiW> at lam, the outputs' source
+        bool ub_lam = true;
+        double ng;
+        if (a.grad_at_lam) {
+            // the residual check's gradient at lambda (its projection summed
+            // in point order); Lambda at the points for the first Hessian
+            if (tid < Q) point_lambda(lam);
+            double sq = 0.0;
+            if (tid < n) {
+                const int l = tid / nt, i = tid - l * nt;
+                const double gv = a.grad_at_lam[cell * SM + i * S + l];
+                g[tid] = gv;
+                sq = gv * gv;
+            }
+            ng = sqrt(block_allsum(sq, red));
+            ub_lam = false;
+        } else {
+            ng = gradient(lam, g);
+        }
         int status = -1;
         int iters = 0;
         if (!c.safe && ng < c.eps) status = UQK_STATUS_CONVERGED;
@@ -761,6 +784,7 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
             }
             __syncthreads();
             ++ngrad;
+            ub_lam = false;
             return gradient(lamnew, gnew);
         };
         for (int it = 0; status < 0 && it < c.max_iter; ++it) {
@@ -811,6 +835,7 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                         double* t = lam;
                         lam = lamnew;
                         lamnew = t;
+                        ub_lam = true;
                         const double ratio = nnew / ng;
                         ng = nnew;
                         // contraction clearly worse than a (damped) Newton
@@ -843,6 +868,7 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                     lam[i * S + l] -= c.alpha * dv[tid];
                 }
                 __syncthreads();
+                ub_lam = false;
                 status = UQK_STATUS_CONVERGED;
                 break;
             }
@@ -869,12 +895,14 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                     double* t = lam;
                     lam = lamnew;
                     lamnew = t;
+                    ub_lam = true;
                 }
                 break;
             }
             double* t = lam;
             lam = lamnew;
             lamnew = t;
+            ub_lam = true;
             ng = nnew;
             if (!isfinite(nnew)) {
                 status = UQK_STATUS_NONFINITE;
@@ -883,6 +911,13 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
             if (nnew < c.eps) status = UQK_STATUS_CONVERGED;
         }
         if (status < 0) status = UQK_STATUS_MAX_ITERATIONS;
+        if (a.uq_out && !ub_lam) {
+            // the outputs come from U at the returned lambda: evaluated again
+            // where the last gradient was another point's (a rejected step, a
+            // failed cell, or the check's gradient alone)
+            ++ngrad;
+            gradient(lam, gnew);
+        }
         if constexpr (PREC == 2) {
             // the slot for the next steps: this call's last factor U (in H),
             // rounded to FP32 and tagged with the cell's order; the first
@@ -901,8 +936,8 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
         }
         for (int t = tid; t < SM; t += kBlockThreads) lg[t] = lam[t];
         if (a.uq_out) {
-            // the step's next two sweeps from the last gradient, which is at
-            // the returned lambda: uQ = U(lambda phi) (momentsolver.cpp:100)
+            // the step's next two sweeps from the gradient at the returned
+            // lambda: uQ = U(lambda phi) (momentsolver.cpp:100)
             // and u = <uQ phiW> over the cell's nTotal, zero above (:120-124)
             double* uqg = a.uq_out + cell * (long)Q * S;
             for (int t = tid; t < Q * S; t += kBlockThreads) {

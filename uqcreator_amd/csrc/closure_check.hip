@@ -124,7 +124,8 @@ __global__ void __launch_bounds__(kCheckThreads) closure_check_kernel(NewtonArgs
 // (a.lam_alt), a cell that fails at its lambda is also evaluated at the
 // network's candidate, which replaces lambda when its ||grad L|| is smaller
 // (a NaN norm never wins), as the wave kernel chooses; a candidate that
-// passes finishes the cell.
+// passes finishes the cell.  The gradients of the listed cells at the duals
+// they keep are stored for the workgroup kernel, which starts from them.
 template <int KIND>
 __global__ void __launch_bounds__(512) closure_check_wave_kernel(NewtonArgs a) {
     constexpr int S = KindInfo<KIND>::S;
@@ -204,6 +205,7 @@ __global__ void __launch_bounds__(512) closure_check_wave_kernel(NewtonArgs a) {
             return nrm;
         };
         double ng = eval(lg);
+        double gk[2] = {acc[0], acc[1]};  // <U phiW> at the duals the cell keeps
         if (!(ng < c.eps) && a.lam_alt) {
             const double* la = a.lam_alt + cell * SM;
             const double nalt = eval(la);
@@ -212,9 +214,19 @@ __global__ void __launch_bounds__(512) closure_check_wave_kernel(NewtonArgs a) {
                 for (int t = lane; t < SM; t += 64) lg[t] = la[t];
                 ng = nalt;
                 ++w_took;
+                gk[0] = acc[0];
+                gk[1] = acc[1];
             }
         }
         if (!(ng < c.eps)) {
+            if (a.grad_at_lam) {
+                // the gradient the Newton kernel starts from
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int e = lane + 64 * h;
+                    if (e < S * nt) a.grad_at_lam[cell * SM + e] = gk[h] - ug[e];
+                }
+            }
             if (lane == 0) pend[np] = (int)cell;
             ++np;
             if (np == 64) flush();

@@ -153,10 +153,9 @@ int ensure_factor_buffers(uqk_ctx* ctx, long n_cells, int mode) {
     return 0;
 }
 
-// The list of cells the residual check leaves to the Newton kernel and, for
-// the wave kernel (grad), their gradients at lambda: grown to n cells,
-// outside graph capture only.
-int ensure_check_list(uqk_ctx* ctx, long n_cells, bool grad) {
+// The list of cells the residual check leaves to the Newton kernel and their
+// gradients at lambda: grown to n cells, outside graph capture only.
+int ensure_check_list(uqk_ctx* ctx, long n_cells) {
     const size_t cells = (size_t)(n_cells > 0 ? n_cells : 1);
     if (ctx->check_cap < n_cells) {
         if (ctx->capturing) return fail(ctx, -1, "residual check: the cell list must exist before graph capture");
@@ -166,7 +165,7 @@ int ensure_check_list(uqk_ctx* ctx, long n_cells, bool grad) {
         UQK_HIP(ctx, hipMalloc(&ctx->check_list, sizeof(int) * cells));
         ctx->check_cap = n_cells;
     }
-    if (grad && ctx->check_grad_cap < n_cells) {
+    if (ctx->check_grad_cap < n_cells) {
         if (ctx->capturing) return fail(ctx, -1, "residual check: the gradients must exist before graph capture");
         hipFree(ctx->check_grad);
         ctx->check_grad = nullptr;
@@ -278,7 +277,7 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         // lambda already passes the convergence test and lists the rest, with
         // their gradients, for the wave kernel
         if (fuse && wave_check(ctx, nt != nullptr)) {
-            if (int rc = ensure_check_list(ctx, n, true)) return rc;
+            if (int rc = ensure_check_list(ctx, n)) return rc;
             uqk::NewtonArgs ca = a;
             ca.list_out = ctx->check_list;
             ca.list_count = counters + 8;
@@ -330,14 +329,16 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev0, ctx->stream));
         // the time loop's step: the residual check (one wave per cell) first
         if (check) {
-            if (int rc = ensure_check_list(ctx, n, false)) return rc;
+            if (int rc = ensure_check_list(ctx, n)) return rc;
             uqk::NewtonArgs ca = a;
             ca.lam_alt = lam_alt;
             ca.list_out = ctx->check_list;
             ca.list_count = counters + 8;
+            ca.grad_at_lam = ctx->check_grad;
             UQK_HIP(ctx, uqk::closure_check_wave_launch(ctx->kind, ca, (int)ctx->max_lds, ctx->num_cu, ctx->stream));
             a.cell_list = ctx->check_list;
             a.cell_count = counters + 8;
+            a.grad_at_lam = ctx->check_grad;
         }
         UQK_HIP(ctx, uqk::newton_block_launch(ctx->kind, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream, ctx->shape));
         if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev1, ctx->stream));
@@ -896,7 +897,7 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
         if (mesh->n_cells > 0)
             if (int rc = ensure_factor_buffers(ctx, mesh->n_cells, fmode)) return rc;
     if (ctx->residual_check && mesh->n_cells > 0)
-        if (int rc = ensure_check_list(ctx, mesh->n_cells, wave_check(ctx, nt != nullptr))) return rc;
+        if (int rc = ensure_check_list(ctx, mesh->n_cells)) return rc;
     UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *ctx->h_run = uqk::RunState{};
     ctx->h_run->t = t0;

@@ -74,7 +74,7 @@ struct uqk_ctx {
     bool residual_check = true;
     int* check_list = nullptr;
     long check_cap = 0;
-    // and, before the wave kernel, their gradients at lambda (n x S x M)
+    // and their gradients at lambda (n x S x M)
     double* check_grad = nullptr;
     long check_grad_cap = 0;
     long pinv_cell_cap = 0;

@@ -53,7 +53,7 @@ struct NewtonArgs {
     const unsigned long long* cell_count;
     // and, when not null, the check kernel stores the gradient at lambda of
     // every cell it lists (cell-major, entry (i, s) at i * S + s), which the
-    // wave kernel takes in place of evaluating it again
+    // Newton kernel takes in place of evaluating it again
     double* grad_at_lam;
     // mixed precision, wave kernel: reuse of the factorisation.
     // pinv_mode 1 (time loop, across steps): a cell whose pinv_ok is 1 holds
