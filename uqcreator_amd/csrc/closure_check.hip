@@ -30,6 +30,7 @@ __global__ void __launch_bounds__(kCheckThreads) closure_check_kernel(NewtonArgs
     __shared__ double tphi[Q * M];   // phi(k, i) at k * M + i
     __shared__ double tphiw[Q * M];
     if (a.active && *a.active == 0) return;
+    if (a.recon_ok && *a.recon_ok) return;  // closure_recheck_kernel's step
     for (int t = threadIdx.x; t < Q * M; t += blockDim.x) {
         tphi[t] = a.phi[t];
         tphiw[t] = a.phiw[t];
@@ -87,6 +88,7 @@ __global__ void __launch_bounds__(kCheckThreads) closure_check_kernel(NewtonArgs
         for (int t = 0; t < N; ++t) uo[t] = acc[t];
         if (a.status) a.status[cs] = UQK_STATUS_CONVERGED;
         if (a.iters) a.iters[cs] = 0;
+        if (a.dt_cell) a.dt_cell[cs] = dtm;
     } else if (valid && a.grad_at_lam) {
         // the gradient the Newton kernel starts from
         double* gg = a.grad_at_lam + cs * N;
@@ -104,15 +106,67 @@ __global__ void __launch_bounds__(kCheckThreads) closure_check_kernel(NewtonArgs
         const unsigned long long below = lane ? (left & (~0ull >> (64 - lane))) : 0ull;
         a.list_out[base + __popcll(below)] = (int)cell;
     }
-    // counters: one gradient per finished cell; ComputeDt of finished cells
-    const unsigned long long done = __ballot(conv);
+    // counters: one gradient per cell (the listed cells' is their Newton
+    // start); ComputeDt of finished cells
+    const unsigned long long evaluated = __ballot(valid);
     double m = conv ? dtm : 1e10;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
     if (lane == 0) {
-        if (done) atomicAdd(a.nfail + 2, (unsigned long long)__popcll(done));
+        if (evaluated) atomicAdd(a.nfail + 2, (unsigned long long)__popcll(evaluated));
         if (a.dt_min && m < 1e10) atomicMin(a.dt_min, (unsigned long long)__double_as_longlong(m));
     }
+}
+
+// The residual check of a run's later steps: the last step's closure left,
+// for every cell, the moments <U(lambda phi) phiW> of the current lambda in
+// u_out, uQ in uq_out and the cell's ComputeDt in dt_cell, so the gradient
+// at lambda is u_out - u without any evaluation of U, and a cell that
+// passes keeps those outputs.  One lane per cell, as closure_check_kernel,
+// whose gradient it equals (the same differences, the same sum of squares);
+// it runs when *recon_ok is set, closure_check_kernel otherwise.
+template <int KIND, int M, int Q>
+__global__ void __launch_bounds__(kCheckThreads) closure_recheck_kernel(NewtonArgs a) {
+    constexpr int S = KindInfo<KIND>::S, N = S * M;
+    if (a.active && *a.active == 0) return;
+    if (!*a.recon_ok) return;
+    const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = cell < a.ncell;
+    const long cs = valid ? cell : 0;
+    const ClosureParams& c = a.c;
+    const double* ur = a.u_out + cs * N;
+    const double* ug = a.u + cs * N;
+    double g[N];
+    double sq = 0.0;
+#pragma unroll
+    for (int t = 0; t < N; ++t) {
+        g[t] = ur[t] - ug[t];
+        sq += g[t] * g[t];
+    }
+    const bool conv = valid && sqrt(sq) < c.eps;
+    if (conv) {
+        if (a.status) a.status[cs] = UQK_STATUS_CONVERGED;
+        if (a.iters) a.iters[cs] = 0;
+    } else if (valid && a.grad_at_lam) {
+        double* gg = a.grad_at_lam + cs * N;
+#pragma unroll
+        for (int t = 0; t < N; ++t) gg[t] = g[t];
+    }
+    const unsigned long long left = __ballot(valid && !conv);
+    const int lane = threadIdx.x & 63;
+    unsigned long long base = 0;
+    if (lane == 0 && left) base = atomicAdd(a.list_count, (unsigned long long)__popcll(left));
+    base = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(base >> 32)) << 32) |
+           (unsigned)__builtin_amdgcn_readfirstlane((unsigned)base);
+    if (valid && !conv) {
+        const unsigned long long below = lane ? (left & (~0ull >> (64 - lane))) : 0ull;
+        a.list_out[base + __popcll(below)] = (int)cell;
+    }
+    // ComputeDt of the finished cells, as the last step found it
+    double m = (conv && a.dt_min) ? a.dt_cell[cs] : 1e10;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
+    if (lane == 0 && a.dt_min && m < 1e10) atomicMin(a.dt_min, (unsigned long long)__double_as_longlong(m));
 }
 
 // The same check for the workgroup kernel's sizes (n = S*M up to 128, any
@@ -266,8 +320,13 @@ static bool check_try(int kind, int m, int q, const NewtonArgs& a, int num_cu, h
     if (kind != KIND || m != M || q != Q) return false;
     (void)num_cu;
     const long blocks = (a.ncell + kCheckThreads - 1) / kCheckThreads;
-    hipLaunchKernelGGL((closure_check_kernel<KIND, M, Q>), dim3((unsigned)(blocks > 0 ? blocks : 1)),
-                       dim3(kCheckThreads), 0, st, a);
+    const dim3 grid((unsigned)(blocks > 0 ? blocks : 1));
+    // (in a run, one of the two kernels works, as the run's flag says)
+    if (a.recon_ok) {
+        hipLaunchKernelGGL((closure_recheck_kernel<KIND, M, Q>), grid, dim3(kCheckThreads), 0, st, a);
+        if ((*err = hipGetLastError()) != hipSuccess) return true;
+    }
+    hipLaunchKernelGGL((closure_check_kernel<KIND, M, Q>), grid, dim3(kCheckThreads), 0, st, a);
     *err = hipGetLastError();
     return true;
 }

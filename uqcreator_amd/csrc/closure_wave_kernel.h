@@ -670,7 +670,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
         double* lam = lamA;
         double* lamnew = lamB;
         double g = 0.0, gnew = 0.0, d = 0.0;
-        int ngrad = 1;
+        // gradients evaluated here (the residual check counts the one it gives)
+        int ngrad = a.grad_at_lam ? 0 : 1;
         // Ub and uproj hold U and <U phiW> at lam, the outputs' source
         bool ub_lam = true;
         double ng;
@@ -884,7 +885,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) n
                     for (int s = 0; s < S; ++s) uk[s] = Ub[s * QP + lane];
                     m = point_dt<S>(a.dtp, uk, dx_c);
                 }
-                w_dt = fmin(w_dt, wave_min(m));
+                const double mc = wave_min(m);
+                w_dt = fmin(w_dt, mc);
+                if (a.dt_cell && lane == 0) a.dt_cell[cell] = mc;
             }
         }
         if (lane == 0) {

@@ -382,10 +382,11 @@ __global__ void __launch_bounds__(kSweepThreads) mean_var_kernel(SweepArgs a) {
 // t < t_end and steps < max_steps and no failed cell: the step runs
 // The end of a replayed step: residual and t (momentsolver.cpp:134-152), run
 // totals, and what the next step's launches need (zero closure counters, the
-// start value 1e10 of ComputeDt's MIN, the active flag).
+// start value 1e10 of ComputeDt's MIN, the active flag; set_recon: the
+// step's closure left its outputs at the current lambda for every cell).
 __global__ void __launch_bounds__(kSweepThreads) step_end_kernel(RunState* rs, double* scalars,
                                                                  unsigned long long* counters,
-                                                                 const double* partial, int npartial) {
+                                                                 const double* partial, int npartial, int set_recon) {
     if (!rs->active) return;
     double m = 0.0;
     for (int t = threadIdx.x; t < npartial; t += blockDim.x) m += partial[t];
@@ -406,13 +407,14 @@ __global__ void __launch_bounds__(kSweepThreads) step_end_kernel(RunState* rs, d
         for (int i = 0; i < kClosureCounters; ++i) counters[i] = 0ull;
         scalars[0] = 1e10;
         rs->active = (t < rs->t_end && (rs->max_steps < 0 || steps < rs->max_steps) && nfail == 0) ? 1 : 0;
+        if (set_recon) rs->recon = 1;
     }
 }
 
 hipError_t launch_step_end(RunState* rs, double* scalars, unsigned long long* counters, const double* partial,
-                           int npartial, hipStream_t st) {
+                           int set_recon, int npartial, hipStream_t st) {
     hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(kSweepThreads), 0, st, rs, scalars, counters, partial,
-                       npartial);
+                       npartial, set_recon);
     return hipGetLastError();
 }
 

@@ -171,6 +171,9 @@ int ensure_check_list(uqk_ctx* ctx, long n_cells) {
         ctx->check_grad = nullptr;
         ctx->check_grad_cap = 0;
         UQK_HIP(ctx, hipMalloc(&ctx->check_grad, sizeof(double) * (size_t)ctx->S * ctx->M * cells));
+        hipFree(ctx->check_dt);
+        ctx->check_dt = nullptr;
+        UQK_HIP(ctx, hipMalloc(&ctx->check_dt, sizeof(double) * cells));
         ctx->check_grad_cap = n_cells;
     }
     return 0;
@@ -198,10 +201,12 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
                    int32_t* iters, const int* active = nullptr, double* uq_out = nullptr, double* u_out = nullptr,
                    bool* fused = nullptr, double* dt_out = nullptr, const uqk::ProblemParams* dtp = nullptr,
                    const double* dx_param = nullptr, bool* fused_dt = nullptr,
-                   unsigned long long* counters = nullptr, bool prezeroed = false, bool factors_per_cell = false) {
+                   unsigned long long* counters = nullptr, bool prezeroed = false, bool factors_per_cell = false,
+                   const int* recon_ok = nullptr) {
     if (fused) *fused = false;
     if (fused_dt) *fused_dt = false;
     ctx->timed = false;
+    ctx->step_recon = false;
     if (!counters) counters = ctx->d_counters;
     // (in a replayed run the previous step's step_end_kernel already cleared
     // the counters and set dt's start value: prezeroed)
@@ -282,11 +287,17 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
             ca.list_out = ctx->check_list;
             ca.list_count = counters + 8;
             ca.grad_at_lam = ctx->check_grad;
+            // in a run, the steps after the first take the gradient from the
+            // last step's outputs (closure_recheck_kernel), which needs every
+            // cell's ComputeDt kept
+            ca.recon_ok = want_dt ? recon_ok : nullptr;
+            ca.dt_cell = ca.recon_ok ? ctx->check_dt : nullptr;
             if (uqk::closure_check_launch(ctx->kind, ctx->M, ctx->Q, ca, ctx->num_cu, ctx->stream, &e)) {
                 UQK_HIP(ctx, e);
                 a.cell_list = ctx->check_list;
                 a.cell_count = counters + 8;
                 a.grad_at_lam = ctx->check_grad;
+                a.dt_cell = ca.dt_cell;
             }
         }
         if (uqk::newton_wave_launch(ctx->kind, ctx->M, ctx->Q, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream, &e,
@@ -301,6 +312,8 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
                 UQK_HIP(ctx, e);
             }
             if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev1, ctx->stream));
+            // every cell's uQ, moments and ComputeDt are now those of its lambda
+            ctx->step_recon = a.dt_cell != nullptr;
             ctx->last_kernel = UQK_KERNEL_WAVE;
             ctx->timed = ev;
             if (fused) *fused = fuse;
@@ -312,6 +325,7 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         a.cell_list = nullptr;
         a.cell_count = nullptr;
         a.grad_at_lam = nullptr;
+        a.dt_cell = nullptr;
     }
     const bool block_ok = uqk::newton_block_fits(ctx->S, ctx->M, ctx->Q, (int)ctx->max_lds);
     if (ctx->kernel == UQK_KERNEL_BLOCK || (ctx->kernel == UQK_KERNEL_AUTO && ctx->S * ctx->M >= 64 && block_ok)) {
@@ -420,7 +434,7 @@ int enqueue_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, in
     }
     int rc = launch_closure(ctx, n, nt, u, lam, status, iters, active, refining ? nullptr : uq,
                             refining ? nullptr : u_scratch, &fused, ctx->d_scalars, &pp, mesh->dx_param, &fused_dt,
-                            nullptr, in_run, true);
+                            nullptr, in_run, true, in_run ? &ctx->d_run->recon : nullptr);
     ctx->cev0 = ctx->cev1 = nullptr;
     if (rc) return rc;
     uqk::SweepArgs a = sweep_args(ctx);
@@ -608,6 +622,7 @@ int uqk_destroy(uqk_ctx* ctx) {
     hipFree(ctx->pinv_list);
     hipFree(ctx->check_list);
     hipFree(ctx->check_grad);
+    hipFree(ctx->check_dt);
     if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
     hipFree(ctx->d_run);
     if (ctx->h_run) hipHostFree(ctx->h_run);
@@ -914,7 +929,7 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
         int rc = enqueue_step(ctx, prob, mesh, nt, u, lam, uq, u_scratch, status, iters, active, true);
         if (rc) return rc;
         UQK_HIP(ctx, uqk::launch_step_end(ctx->d_run, ctx->d_scalars, ctx->d_counters, ctx->d_partial,
-                                          uqk::sweep_partial_blocks(0), ctx->stream));
+                                          ctx->step_recon ? 1 : 0, uqk::sweep_partial_blocks(0), ctx->stream));
         return 0;
     };
     if (use_graph) {

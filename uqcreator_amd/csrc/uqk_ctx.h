@@ -74,9 +74,14 @@ struct uqk_ctx {
     bool residual_check = true;
     int* check_list = nullptr;
     long check_cap = 0;
-    // and their gradients at lambda (n x S x M)
+    // and their gradients at lambda (n x S x M), and every cell's ComputeDt
+    // (the steps of a run after the first)
     double* check_grad = nullptr;
+    double* check_dt = nullptr;
     long check_grad_cap = 0;
+    // the last launch_closure left uQ, the moments and each cell's ComputeDt
+    // at lambda for every cell (a run's step then sets RunState::recon)
+    bool step_recon = false;
     long pinv_cell_cap = 0;
     std::string err;
 };

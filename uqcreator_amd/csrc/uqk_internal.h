@@ -55,6 +55,14 @@ struct NewtonArgs {
     // every cell it lists (cell-major, entry (i, s) at i * S + s), which the
     // Newton kernel takes in place of evaluating it again
     double* grad_at_lam;
+    // time loop, wave kernel shapes: each cell's ComputeDt (the fused
+    // kernels store it), and the run's flag that the last step's outputs
+    // (uQ, u_out, dt_cell) are those of the current lambda -- then the
+    // residual check needs no evaluation of U: the gradient at lambda is
+    // u_out - u (u_out = <U(lambda phi) phiW>), and a cell that passes keeps
+    // its uQ, moments and ComputeDt
+    double* dt_cell;
+    const int* recon_ok;
     // mixed precision, wave kernel: reuse of the factorisation.
     // pinv_mode 1 (time loop, across steps): a cell whose pinv_ok is 1 holds
     // a kept inverse Hessian in pinv (FP32, n x n, row stride n + 1) and
@@ -92,6 +100,10 @@ struct RunState {
     double t_end;
     long long max_steps;  // < 0: no limit
     int active;
+    // 1 once a step's closure has left uQ, the reconstructed moments and
+    // every cell's ComputeDt at the current lambda (zero at the run's start):
+    // the next step's residual check then takes the gradient from them
+    int recon;
 };
 
 // Closure counters of a launch: [0] work-queue head, [1] failed cells, [2]
@@ -215,6 +227,6 @@ int sweep_partial_blocks(long ncell);
 // run totals; then clears the closure counters, sets the fused dt's start value
 // and the next step's active flag.
 hipError_t launch_step_end(RunState* rs, double* scalars, unsigned long long* counters, const double* partial,
-                           int npartial, hipStream_t st);
+                           int set_recon, int npartial, hipStream_t st);
 
 }  // namespace uqk
