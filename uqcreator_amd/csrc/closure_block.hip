@@ -753,7 +753,8 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
 
         double* g = gA;
         double* gnew = gB;
-        int ngrad = 1;
+        // gradients evaluated here (the residual check counts the one it gives)
+        int ngrad = a.grad_at_lam ? 0 : 1;
         // Ub and uproj hold U and <U phiW> at lam, the outputs' source
         bool ub_lam = true;
         double ng;
@@ -955,7 +956,13 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                     for (int sx = 0; sx < S; ++sx) uk[sx] = Ub[sx * Q + tid];
                     m = point_dt<S>(a.dtp, uk, a.dx_param[cell]);
                 }
-                w_dt = fmin(w_dt, wave_min(m));  // per wavefront; combined below
+                const double wm = wave_min(m);
+                w_dt = fmin(w_dt, wm);  // per wavefront; combined below
+                if (a.dt_cell) {  // the cell's ComputeDt, for the next step's check
+                    if (lane == 0) red[4 + wave] = wm;
+                    __syncthreads();
+                    if (tid == 0) a.dt_cell[cell] = fmin(fmin(red[4], red[5]), fmin(red[6], red[7]));
+                }
             }
         }
         if (tid == 0) {

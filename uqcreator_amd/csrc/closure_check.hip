@@ -179,7 +179,9 @@ __global__ void __launch_bounds__(kCheckThreads) closure_recheck_kernel(NewtonAr
 // network's candidate, which replaces lambda when its ||grad L|| is smaller
 // (a NaN norm never wins), as the wave kernel chooses; a candidate that
 // passes finishes the cell.  The gradients of the listed cells at the duals
-// they keep are stored for the workgroup kernel, which starts from them.
+// they keep are stored for the workgroup kernel, which starts from them.  In
+// a run's later steps (*recon_ok) the gradient at lambda comes from the last
+// step's moments, as in closure_recheck_kernel.
 template <int KIND>
 __global__ void __launch_bounds__(512) closure_check_wave_kernel(NewtonArgs a) {
     constexpr int S = KindInfo<KIND>::S;
@@ -209,8 +211,12 @@ __global__ void __launch_bounds__(512) closure_check_wave_kernel(NewtonArgs a) {
         np = 0;
         wave_sync();
     };
-    unsigned long long w_done = 0, w_took = 0, w_alt = 0;
+    unsigned long long w_done = 0, w_took = 0, w_alt = 0, w_eval = 0;
     double w_dt = 1e10;
+    // a run's later steps: the last step left every cell's outputs at its
+    // lambda (u_out = <U(lambda phi) phiW>, uQ, dt_cell), so the gradient at
+    // lambda is u_out - u, and a cell that passes there keeps them
+    const bool recon = a.recon_ok && *a.recon_ok;
     const long stride = (long)gridDim.x * nw;
     for (long cell = (long)blockIdx.x * nw + wave; cell < a.ncell; cell += stride) {
         const int nt = a.nt_cell ? a.nt_cell[cell] : M;
@@ -258,7 +264,26 @@ __global__ void __launch_bounds__(512) closure_check_wave_kernel(NewtonArgs a) {
             wave_sync();
             return nrm;
         };
-        double ng = eval(lg);
+        // ||grad L|| at lambda from the last step's moments (recon)
+        auto from_outputs = [&]() -> double {
+            const double* uo = a.u_out + cell * SM;
+            double sq = 0.0;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int e = lane + 64 * h;
+                acc[h] = 0.0;
+                if (e < S * nt) {
+                    const double v = uo[e];
+                    acc[h] = v;
+                    const double g = v - ug[e];
+                    sq += g * g;
+                }
+            }
+            return sqrt(wave_sum(sq));
+        };
+        double ng = recon ? from_outputs() : eval(lg);
+        if (!recon) ++w_eval;
+        bool kept = recon;  // the cell's outputs stay the last step's
         double gk[2] = {acc[0], acc[1]};  // <U phiW> at the duals the cell keeps
         if (!(ng < c.eps) && a.lam_alt) {
             const double* la = a.lam_alt + cell * SM;
@@ -268,6 +293,7 @@ __global__ void __launch_bounds__(512) closure_check_wave_kernel(NewtonArgs a) {
                 for (int t = lane; t < SM; t += 64) lg[t] = la[t];
                 ng = nalt;
                 ++w_took;
+                kept = false;
                 gk[0] = acc[0];
                 gk[1] = acc[1];
             }
@@ -287,18 +313,26 @@ __global__ void __launch_bounds__(512) closure_check_wave_kernel(NewtonArgs a) {
             continue;
         }
         ++w_done;
-        double* uqg = a.uq_out + cell * (long)Q * S;
-        for (int t = lane; t < Q * S; t += 64) {
-            const int k = t / S, s = t - k * S;
-            uqg[t] = Ub[s * Q + k];
-        }
-        double* uo = a.u_out + cell * SM;
+        if (kept) {
+            if (a.dt_min) w_dt = fmin(w_dt, a.dt_cell[cell]);
+        } else {
+            double* uqg = a.uq_out + cell * (long)Q * S;
+            for (int t = lane; t < Q * S; t += 64) {
+                const int k = t / S, s = t - k * S;
+                uqg[t] = Ub[s * Q + k];
+            }
+            double* uo = a.u_out + cell * SM;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int e = lane + 64 * h;
-            if (e < SM) uo[e] = e < S * nt ? acc[h] : 0.0;
+            for (int h = 0; h < 2; ++h) {
+                const int e = lane + 64 * h;
+                if (e < SM) uo[e] = e < S * nt ? acc[h] : 0.0;
+            }
+            if (a.dt_min) {
+                const double mc = wave_min(m);
+                w_dt = fmin(w_dt, mc);
+                if (a.dt_cell && lane == 0) a.dt_cell[cell] = mc;
+            }
         }
-        if (a.dt_min) w_dt = fmin(w_dt, wave_min(m));
         if (lane == 0) {
             if (a.status) a.status[cell] = UQK_STATUS_CONVERGED;
             if (a.iters) a.iters[cell] = 0;
@@ -307,9 +341,9 @@ __global__ void __launch_bounds__(512) closure_check_wave_kernel(NewtonArgs a) {
     }
     if (np > 0) flush();
     if (lane == 0) {
-        // gradients: the finished cells' and every candidate's (the listed
-        // cells' first gradient counts in the Newton kernel)
-        if (w_done + w_alt) atomicAdd(a.nfail + 2, w_done + w_alt);
+        // gradients evaluated: at lambda (not from the last step's moments)
+        // and at every candidate
+        if (w_eval + w_alt) atomicAdd(a.nfail + 2, w_eval + w_alt);
         if (w_took && a.took) atomicAdd(a.took, w_took);
         if (a.dt_min && w_dt < 1e10) atomicMin(a.dt_min, (unsigned long long)__double_as_longlong(w_dt));
     }

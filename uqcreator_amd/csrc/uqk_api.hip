@@ -349,13 +349,19 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
             ca.list_out = ctx->check_list;
             ca.list_count = counters + 8;
             ca.grad_at_lam = ctx->check_grad;
+            // in a run, the steps after the first take the gradient from the
+            // last step's outputs, which needs every cell's ComputeDt kept
+            ca.recon_ok = want_dt ? recon_ok : nullptr;
+            ca.dt_cell = ca.recon_ok ? ctx->check_dt : nullptr;
             UQK_HIP(ctx, uqk::closure_check_wave_launch(ctx->kind, ca, (int)ctx->max_lds, ctx->num_cu, ctx->stream));
             a.cell_list = ctx->check_list;
             a.cell_count = counters + 8;
             a.grad_at_lam = ctx->check_grad;
+            a.dt_cell = ca.dt_cell;
         }
         UQK_HIP(ctx, uqk::newton_block_launch(ctx->kind, a, (int)ctx->max_lds, ctx->num_cu, ctx->stream, ctx->shape));
         if (ev) UQK_HIP(ctx, hipEventRecord(ctx->tev1, ctx->stream));
+        ctx->step_recon = a.dt_cell != nullptr;
         ctx->last_kernel = UQK_KERNEL_BLOCK;
         ctx->timed = ev;
         if (fused) *fused = fuse;
