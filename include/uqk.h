@@ -276,8 +276,14 @@ int uqk_set_factor_reuse(uqk_ctx* ctx, int32_t on);
  * kernel's shapes, one wave per cell for the workgroup kernel's (n up to
  * 128, per-cell orders too) -- finishes the cells that pass (uQ, the
  * moments, ComputeDt, status, zero iterations) and lists the others, which
- * the Newton kernel then takes.  Lambda and U at the points are the Newton
- * kernel's; the results agree with the check off to rounding. */
+ * the Newton kernel then takes, starting from the gradient the check
+ * found.  Within one uqk_run_steps call, the steps after the first take
+ * that gradient from the previous step's outputs instead (the moments
+ * <U(lambda phi) phiW> it left in u_scratch, minus u), and a cell that
+ * passes keeps its uQ, moments and ComputeDt: the first step of every call
+ * evaluates U, since the caller may have changed u, lambda or u_scratch
+ * between calls.  Lambda and U at the points are the Newton kernel's; the
+ * results agree with the check off to rounding. */
 int uqk_set_residual_check(uqk_ctx* ctx, int32_t on);
 int uqk_invalidate_factors(uqk_ctx* ctx);
 /* Cholesky factorisations and steps on a kept inverse of the last closure
