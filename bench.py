@@ -123,10 +123,12 @@ class C3:
         st = s.status.download()
         fm = flop_model(4, self.b.M, self.b.Q)
         S, M, Q = 4, self.b.M, self.b.Q
-        # the mixed time loop with kept inverses runs instantiation 2 and the
-        # inverse kernel, both inside the closure's timing window
-        kname = ("newton_wave_kernel<2,10,25,2> + factor_inverse_kernel<40>" if lagged > 0 else
-                 "newton_wave_kernel<2,10,25,%d>" % precision)
+        # the closure's timing window: the residual check (from the last
+        # step's moments after the run's first step), the Newton kernel and,
+        # in the mixed time loop with kept inverses, the inverse kernel
+        kname = ("closure_recheck_kernel<2,10,25> (closure_check_kernel<2,10,25> in the first step) + " +
+                 ("newton_wave_kernel<2,10,25,2> + factor_inverse_kernel<40>" if lagged > 0 else
+                  "newton_wave_kernel<2,10,25,%d>" % precision))
         roof_c = closure_roofline(kname, closure_ms / timed_steps, newton / args.steps, grads / args.steps, fm,
                                   precision == native.MIXED, factors / args.steps, lagged / args.steps)
         adv_ms = advance_ms / timed_steps
