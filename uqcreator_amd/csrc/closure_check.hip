@@ -30,7 +30,6 @@ __global__ void __launch_bounds__(kCheckThreads) closure_check_kernel(NewtonArgs
     __shared__ double tphi[Q * M];   // phi(k, i) at k * M + i
     __shared__ double tphiw[Q * M];
     if (a.active && *a.active == 0) return;
-    if (a.recon_ok && *a.recon_ok) return;  // closure_recheck_kernel's step
     for (int t = threadIdx.x; t < Q * M; t += blockDim.x) {
         tphi[t] = a.phi[t];
         tphiw[t] = a.phiw[t];
@@ -123,8 +122,9 @@ __global__ void __launch_bounds__(kCheckThreads) closure_check_kernel(NewtonArgs
 // u_out, uQ in uq_out and the cell's ComputeDt in dt_cell, so the gradient
 // at lambda is u_out - u without any evaluation of U, and a cell that
 // passes keeps those outputs.  One lane per cell, as closure_check_kernel,
-// whose gradient it equals (the same differences, the same sum of squares);
-// it runs when *recon_ok is set, closure_check_kernel otherwise.
+// whose gradient it equals (the same differences, the same sum of squares).
+// The host launches it for a run's steps after the first; the run's flag
+// (set when a step has left those outputs) guards it.
 template <int KIND, int M, int Q>
 __global__ void __launch_bounds__(kCheckThreads) closure_recheck_kernel(NewtonArgs a) {
     constexpr int S = KindInfo<KIND>::S, N = S * M;
@@ -355,12 +355,11 @@ static bool check_try(int kind, int m, int q, const NewtonArgs& a, int num_cu, h
     (void)num_cu;
     const long blocks = (a.ncell + kCheckThreads - 1) / kCheckThreads;
     const dim3 grid((unsigned)(blocks > 0 ? blocks : 1));
-    // (in a run, one of the two kernels works, as the run's flag says)
-    if (a.recon_ok) {
+    // a run's later steps (recon_ok): the check from the last step's outputs
+    if (a.recon_ok)
         hipLaunchKernelGGL((closure_recheck_kernel<KIND, M, Q>), grid, dim3(kCheckThreads), 0, st, a);
-        if ((*err = hipGetLastError()) != hipSuccess) return true;
-    }
-    hipLaunchKernelGGL((closure_check_kernel<KIND, M, Q>), grid, dim3(kCheckThreads), 0, st, a);
+    else
+        hipLaunchKernelGGL((closure_check_kernel<KIND, M, Q>), grid, dim3(kCheckThreads), 0, st, a);
     *err = hipGetLastError();
     return true;
 }

@@ -287,11 +287,12 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
             ca.list_out = ctx->check_list;
             ca.list_count = counters + 8;
             ca.grad_at_lam = ctx->check_grad;
-            // in a run, the steps after the first take the gradient from the
-            // last step's outputs (closure_recheck_kernel), which needs every
-            // cell's ComputeDt kept
-            ca.recon_ok = want_dt ? recon_ok : nullptr;
-            ca.dt_cell = ca.recon_ok ? ctx->check_dt : nullptr;
+            // in a run every cell's ComputeDt is kept, and the steps after the
+            // first take the gradient from the last step's outputs
+            // (closure_recheck_kernel)
+            const bool keep = want_dt && recon_ok;
+            ca.recon_ok = keep && ctx->check_mode == 2 ? recon_ok : nullptr;
+            ca.dt_cell = keep ? ctx->check_dt : nullptr;
             if (uqk::closure_check_launch(ctx->kind, ctx->M, ctx->Q, ca, ctx->num_cu, ctx->stream, &e)) {
                 UQK_HIP(ctx, e);
                 a.cell_list = ctx->check_list;
@@ -349,10 +350,11 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
             ca.list_out = ctx->check_list;
             ca.list_count = counters + 8;
             ca.grad_at_lam = ctx->check_grad;
-            // in a run, the steps after the first take the gradient from the
-            // last step's outputs, which needs every cell's ComputeDt kept
-            ca.recon_ok = want_dt ? recon_ok : nullptr;
-            ca.dt_cell = ca.recon_ok ? ctx->check_dt : nullptr;
+            // in a run every cell's ComputeDt is kept, and the steps after the
+            // first take the gradient from the last step's outputs
+            const bool keep = want_dt && recon_ok;
+            ca.recon_ok = keep && ctx->check_mode == 2 ? recon_ok : nullptr;
+            ca.dt_cell = keep ? ctx->check_dt : nullptr;
             UQK_HIP(ctx, uqk::closure_check_wave_launch(ctx->kind, ca, (int)ctx->max_lds, ctx->num_cu, ctx->stream));
             a.cell_list = ctx->check_list;
             a.cell_count = counters + 8;
@@ -931,34 +933,48 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
     UQK_HIP(ctx, uqk::launch_fill_u64(ctx->d_counters, uqk::kClosureCounters, ctx->stream));
     UQK_HIP(ctx, uqk::launch_fill_f64(ctx->d_scalars, 1, 1e10, ctx->stream));
     const int* active = &ctx->d_run->active;
-    auto enqueue = [&]() -> int {
+    // check_mode: 1 the residual check evaluates U (the run's first step:
+    // the caller may have changed the state since the last call), 2 it takes
+    // the gradient from the last step's outputs
+    auto enqueue = [&](int check_mode) -> int {
+        ctx->check_mode = check_mode;
         int rc = enqueue_step(ctx, prob, mesh, nt, u, lam, uq, u_scratch, status, iters, active, true);
+        ctx->check_mode = 0;
         if (rc) return rc;
         UQK_HIP(ctx, uqk::launch_step_end(ctx->d_run, ctx->d_scalars, ctx->d_counters, ctx->d_partial,
                                           ctx->step_recon ? 1 : 0, uqk::sweep_partial_blocks(0), ctx->stream));
         return 0;
     };
-    if (use_graph) {
+    const int64_t limit = max_steps < 0 ? INT64_MAX : max_steps;
+    int64_t launched = 0;
+    int later = 1;  // the later steps' check_mode
+    if (limit > 0) {
+        if (int rc = enqueue(1)) return rc;
+        launched = 1;
+        later = ctx->step_recon ? 2 : 1;
+        UQK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (use_graph && launched < limit) {
         // the arguments and the context state the captured launches depend on
         // (the setters of that state also drop the graph)
         char key[640];
         snprintf(key, sizeof key,
                  "%d %d %.17g %.17g %.17g %.17g %d|%lld %d %d %p %p %p %p %p %p %p|%p %p %p %p %p %p %p|"
-                 "%d %d %d %p %p",
+                 "%d %d %d %p %p %d",
                  prob->problem, prob->flux, prob->cfl, prob->gamma, prob->gravity, prob->friction, prob->friction_dim,
                  (long long)mesh->n_cells, mesh->n_edges, mesh->dim, (const void*)mesh->neighbor,
                  (const void*)mesh->unit_normal, (const void*)mesh->normal, (const void*)mesh->area,
                  (const void*)mesh->dx_param, (const void*)mesh->bc, (const void*)mesh->boundary_normal,
                  (const void*)nt, (void*)u, (void*)lam, (void*)uq, (void*)u_scratch, (void*)status, (void*)iters,
                  ctx->kernel, ctx->shape.waves, ctx->shape.blocks_per_cu,
-                 (void*)ctx->warm, (void*)ctx->stream);
+                 (void*)ctx->warm, (void*)ctx->stream, later);
         if (!ctx->graph_exec || ctx->graph_key != key) {
             if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
             ctx->graph_exec = nullptr;
             hipGraph_t g = nullptr;
             UQK_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             ctx->capturing = true;
-            int rc = enqueue();
+            int rc = enqueue(later);
             ctx->capturing = false;
             hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
             if (rc) {
@@ -975,16 +991,14 @@ int64_t uqk_run_steps(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mes
     // replay; every `chunk` steps read the run state back (into pinned host
     // memory owned by the context) to stop early
     uqk::RunState& rs = *ctx->h_run;
-    int64_t launched = 0;
     const int64_t chunk = 64;
-    const int64_t limit = max_steps < 0 ? INT64_MAX : max_steps;
     for (;;) {
         const int64_t todo = std::min<int64_t>(chunk, limit - launched);
         for (int64_t i = 0; i < todo; ++i) {
             if (use_graph) {
                 UQK_HIP(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
             } else {
-                int rc = enqueue();
+                int rc = enqueue(later);
                 if (rc) return rc;
             }
         }

@@ -82,6 +82,9 @@ struct uqk_ctx {
     // the last launch_closure left uQ, the moments and each cell's ComputeDt
     // at lambda for every cell (a run's step then sets RunState::recon)
     bool step_recon = false;
+    // within uqk_run_steps: 1 the step's residual check evaluates U, 2 it
+    // takes the gradient from the last step's outputs; 0 outside a run
+    int check_mode = 0;
     long pinv_cell_cap = 0;
     std::string err;
 };
