@@ -191,8 +191,9 @@ int64_t uqk_step(uqk_ctx* ctx, const uqk_problem* prob, const uqk_mesh* mesh, in
                  double* dt_out, double* residual_out);
 /* The time loop of MomentSolver::Solve (momentsolver.cpp:79-162) on the
  * device: while t < t_end and fewer than max_steps steps (max_steps < 0: no
- * limit) and no closure failure, run the step body.  With use_graph the step is
- * captured once into a hipGraph (re-captured when any argument changes) and
+ * limit) and no closure failure, run the step body.  The call's first step
+ * is enqueued on its own; with use_graph the later steps' body is captured
+ * once into a hipGraph (re-captured when any argument changes) and
  * replayed; t, the step count and failures live on the device and are read
  * back every 64 steps, so there is no host synchronisation inside a step.
  * Returns the number of failed cells of the last step (>= 0) or an error. */
