@@ -24,6 +24,10 @@ and against the oracle after its first steps.  The closure and advance kernels
 are timed with HIP events on the launch stream over the same K steps run again
 without the graph (same trajectory), which gives each kernel's roofline.
 
+The same legs also close every cell once from the plain start (the run's
+first step, lambda_0 = (DS(u_0), 0, ...)), where the learned warm start
+replaces most Newton steps (closure_from_lambda0).
+
 Secondary: the batched-closure benchmark (1D Euler, M = 15, Q = 49, 2^20
 cells, one Newton closure of every cell from lambda_0 = (DS(u_0), 0, ...) to
 ||grad L||_2 <= 1e-10) with the same three legs.
@@ -153,6 +157,33 @@ class C3:
         if net is not None:
             net.detach()
         return out, mv
+
+    def cold_closure(self, precision, reps, net_params=None):
+        """One closure of every cell from the plain start, the run's first
+        step (lambda_0 = (DS(u_0), 0, ...), momentsolver.cpp:463-523 then
+        closure.cpp:133-193), the warm start's prediction and selection
+        included: wall time around the call, best of `reps`."""
+        native, np = self.native, self.np
+        cl, _, net = self.solver(precision, net_params)
+        n = self.n
+        du, dl = native.DeviceArray.from_host(self.u0), native.DeviceArray.from_host(self.lam0)
+        st, it = native.DeviceArray(n, np.int32), native.DeviceArray(n, np.int32)
+        cl.solve(du, dl, st, it)  # warm-up
+        best, nfail = None, 0
+        for _ in range(reps):
+            dl.upload(self.lam0)
+            native.synchronize()
+            t0 = time.perf_counter()
+            nfail = cl.solve(du, dl, st, it)
+            native.synchronize()
+            wall = time.perf_counter() - t0
+            best = wall if best is None else min(best, wall)
+        out = {"cells_per_s": n / best, "wall_ms": 1e3 * best, "failed": int(nfail),
+               "newton_steps_per_cell": cl.last_closure_stats()[0] / n}
+        lam = dl.download()
+        if net is not None:
+            net.detach()
+        return out, lam
 
     def check_vs_oracle(self, oracle_py, precision, steps, lam_o_mv):
         """E/Var rel L2 against the oracle after the first `steps` steps."""
@@ -334,6 +365,7 @@ def main():
     ap.add_argument("--legs", default="fp64,mixed,mixed_warm_start")
     ap.add_argument("--oracle-steps", type=int, default=200,
                     help="steps of the CPU oracle run: the CPU baseline and the check of every leg")
+    ap.add_argument("--cold-reps", type=int, default=5, help="repeats of the closure from lambda_0 (best taken)")
     ap.add_argument("--c3-train-snapshots", type=int, default=8)
     ap.add_argument("--c3-train-steps", type=int, default=19200)
     ap.add_argument("--no-graph", action="store_true",
@@ -394,6 +426,23 @@ def main():
         print("c3 leg %s: %.4g cell-steps/s, %.3f ms/step, closure %.3f ms, advance %.3f ms"
               % (name, legs[name]["value"], legs[name]["ms_per_step"], legs[name]["kernel_ms_per_step"]["closure"],
                  legs[name]["kernel_ms_per_step"]["advance"]), file=sys.stderr, flush=True)
+    # the same legs on one closure of every cell from the plain start (the
+    # run's first step), where the warm start replaces most Newton steps
+    cold, cold_lam = {}, {}
+    for name in legs_wanted:
+        prec = native.FP64 if name == "fp64" else native.MIXED
+        cold[name], cold_lam[name] = c3.cold_closure(prec, args.cold_reps,
+                                                     warm if name == "mixed_warm_start" else None)
+        print("c3 closure from lambda_0, %s: %.4g cells/s, %.3f ms, %.2f Newton steps per cell"
+              % (name, cold[name]["cells_per_s"], cold[name]["wall_ms"], cold[name]["newton_steps_per_cell"]),
+              file=sys.stderr, flush=True)
+    if "fp64" in cold_lam:
+        lref = cold_lam["fp64"]
+        scale = np.maximum(np.abs(lref).max(axis=(1, 2)), 1e-300)
+        for name in cold:
+            cold[name]["lambda_max_rel_diff_vs_fp64"] = float((np.abs(cold_lam[name] - lref).max(axis=(1, 2))
+                                                               / scale).max())
+            cold[name]["speedup_vs_fp64"] = cold[name]["cells_per_s"] / cold["fp64"]["cells_per_s"]
     ref = mvs.get("fp64")
     for name in legs:
         legs[name]["mean_var_rel_l2_vs_fp64"] = rel_l2(mvs[name], ref, c3.m.area) if ref is not None else None
@@ -430,6 +479,9 @@ def main():
         "speedup_vs_fp64": {k: v["value"] / legs["fp64"]["value"] for k, v in legs.items()} if "fp64" in legs
         else None,
         "warm_start_training": train_info,
+        "closure_from_lambda0": {"workload": "one closure of the 65,536 cells from lambda_0 = (DS(u_0), 0, ...), "
+                                             "the run's first step; wall time around the call, best of %d"
+                                             % args.cold_reps, "legs": cold},
         "workload_build_s": build_s,
     }
 
