@@ -840,8 +840,13 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                         const double ratio = nnew / ng;
                         ng = nnew;
                         // contraction clearly worse than a (damped) Newton
-                        // step's: factor afresh at the next Newton step
-                        if (!(ratio <= 1.2 * (1.0 - c.alpha) + 1e-3)) refactor = true;
+                        // step's: factor afresh at the next Newton step.
+                        // From n = 64 on, a factorisation (Hessian O(Q n^2),
+                        // Cholesky O(n^3)) costs over ten steps on the kept
+                        // factor (O(n^2 + Q n)), and a step that contracts
+                        // by 0.1 is still worth keeping
+                        const double slack = n >= 64 ? 0.1 : 1e-3;
+                        if (!(ratio <= 1.2 * (1.0 - c.alpha) + slack)) refactor = true;
                         if (nnew < c.eps) status = UQK_STATUS_CONVERGED;
                     } else {
 #pragma unroll
