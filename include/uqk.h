@@ -257,7 +257,7 @@ int uqk_last_closure_stats(const uqk_ctx* ctx, int64_t* newton_steps, int64_t* g
  * the FP64 Hessian-vector product (d += P (g - H d)) and keep it when the
  * full step lowers ||grad L|| (the FP64 accept test); otherwise, and after
  * a contraction clearly worse than a damped Newton step's (above
- * 1.2 (1 - alpha) + 1e-3; + 0.1 from n = 64 on, where a factorisation costs
+ * 1.2 (1 - alpha) + 1e-3; + 0.05 from n = 64 on, where a factorisation costs
  * over ten steps on the kept one), the cell's current Hessian is factored
  * and the reference Newton step runs.  Memory per cell: n (n + 1) floats and
  * n (n + 3) / 2 doubles (wave kernel), n (n + 1) / 2 floats (workgroup

@@ -844,8 +844,8 @@ __global__ void __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per
                         // From n = 64 on, a factorisation (Hessian O(Q n^2),
                         // Cholesky O(n^3)) costs over ten steps on the kept
                         // factor (O(n^2 + Q n)), and a step that contracts
-                        // by 0.1 is still worth keeping
-                        const double slack = n >= 64 ? 0.1 : 1e-3;
+                        // by 0.05 is still worth keeping
+                        const double slack = n >= 64 ? 0.05 : 1e-3;
                         if (!(ratio <= 1.2 * (1.0 - c.alpha) + slack)) refactor = true;
                         if (nnew < c.eps) status = UQK_STATUS_CONVERGED;
                     } else {
