@@ -48,18 +48,20 @@ def test_refine_pass_matches_oracle(oracle, kind, d, degrees):
     cl.set_refinement(None)
 
 
-def _run_levels_both(oracle, kind, b, m, u0, lam0, prob_kw, clo_kw, steps, above, below, graph):
+def _run_levels_both(oracle, kind, b, m, u0, lam0, prob_kw, clo_kw, steps, above, below, graph, keep=None):
     prob = native.problem(**prob_kw)
     okw = dict(prob_kw)
     oprob = oracle.problem(okw.pop("kind"), **okw)
     cl = native.GpuClosure(kind, b, **clo_kw)
+    if keep is not None:
+        keep["closure"] = cl
     s = solver.GpuMomentSolver(cl, prob, m, levels=True, refine_above=above, coarsen_below=below)
     s.set_state(u0, lam0)
     if graph:
         s.run_device(1e3, steps)
     else:
         s.run(1e3, steps)
-    ocl = {{"epsilon": "eps"}.get(k, k): v for k, v in clo_kw.items()}
+    ocl = {{"epsilon": "eps"}.get(k, k): v for k, v in clo_kw.items() if k != "precision"}
     c = _oracle_closure(oracle, kind, **ocl)
     uo, lo = u0.copy(), lam0.copy()
     nto = np.full(m.n_cells, b.M, np.int32)
@@ -106,6 +108,30 @@ def test_euler2d_levels_time_loop(oracle):
     assert np.array_equal(nt, nto)
     assert len(np.unique(nt)) >= 2
     assert err.max() < 1e-8
+
+
+def test_euler2d_levels_mixed_precision(oracle):
+    """The same loop in mixed precision: per-cell orders (n = 4 nTotal <= 40)
+    take the workgroup kernel, which has their FP32 MFMA Hessian and kept
+    factors (the generic kernel is FP64 only); levels equal to the oracle's
+    FP64 loop cell by cell, E/Var within the mixed bar 1e-6."""
+    b = native.Basis(2, [1, 2, 3], 5)
+    m = meshes.naca_ogrid(64, 16)
+    u0 = workloads.project_ic(workloads.euler2d_freestream_ic(), m.center, b)
+    c = oracle.closure(oracle.EULER2D)
+    lam0 = np.zeros_like(u0)
+    for j in range(m.n_cells):
+        lam0[j, 0] = oracle.DS(c, u0[j, 0])
+    print("case: Euler2D NACA levels, mixed precision", flush=True)
+    keep = {}
+    nt, nto, err = _run_levels_both(oracle, native.EULER2D, b, m, u0, lam0,
+                                    dict(kind=native.EULER2D_PROBLEM, flux=native.FLUX_LAX_FRIEDRICHS, cfl=0.4),
+                                    dict(epsilon=1e-6, precision=native.MIXED), 20, 2e-3, 4e-4, True, keep)
+    print("levels", np.unique(nt, return_counts=True), "rel L2", err, flush=True)
+    assert keep["closure"].last_kernel() == native.KERNEL_BLOCK
+    assert np.array_equal(nt, nto)
+    assert len(np.unique(nt)) >= 2
+    assert err.max() < 1e-6
 
 
 def test_adaptive_input_file(oracle, tmp_path):
