@@ -118,10 +118,10 @@ int uqk_synchronize(uqk_ctx* ctx);
 
 /* Closure kernel selection: AUTO picks the compile-time specialised wave
  * kernel when one exists for (closure, M, Q) and the batch has one order,
- * else the workgroup-per-cell kernel for systems of 64..256 duals and for
- * per-cell orders in mixed precision (whose FP32 MFMA Hessian and kept
- * factors only it has), else the generic wave-per-cell kernel (any size,
- * per-cell orders, FP64). */
+ * else the workgroup-per-cell kernel for systems of 64..256 duals, else the
+ * generic wave-per-cell kernel (any size, per-cell orders, FP64 Hessian;
+ * below 64 duals it outruns the workgroup kernel's mixed precision, which
+ * UQK_KERNEL_BLOCK still selects). */
 enum { UQK_KERNEL_AUTO = 0, UQK_KERNEL_GENERIC = 1, UQK_KERNEL_WAVE = 2, UQK_KERNEL_BLOCK = 3 };
 int uqk_set_closure_kernel(uqk_ctx* ctx, int32_t kernel);
 /* Kernel used by the last closure call (UQK_KERNEL_GENERIC / _WAVE / _BLOCK). */

@@ -54,6 +54,8 @@ def _run_levels_both(oracle, kind, b, m, u0, lam0, prob_kw, clo_kw, steps, above
     oprob = oracle.problem(okw.pop("kind"), **okw)
     cl = native.GpuClosure(kind, b, **clo_kw)
     if keep is not None:
+        if "kernel" in keep:
+            cl.set_kernel(keep["kernel"])
         keep["closure"] = cl
     s = solver.GpuMomentSolver(cl, prob, m, levels=True, refine_above=above, coarsen_below=below)
     s.set_state(u0, lam0)
@@ -111,10 +113,11 @@ def test_euler2d_levels_time_loop(oracle):
 
 
 def test_euler2d_levels_mixed_precision(oracle):
-    """The same loop in mixed precision: per-cell orders (n = 4 nTotal <= 40)
-    take the workgroup kernel, which has their FP32 MFMA Hessian and kept
-    factors (the generic kernel is FP64 only); levels equal to the oracle's
-    FP64 loop cell by cell, E/Var within the mixed bar 1e-6."""
+    """The same loop in mixed precision on the workgroup kernel (chosen: the
+    automatic choice keeps the generic FP64 kernel below n = 64, where it is
+    faster), whose FP32 MFMA Hessian and kept factors then serve per-cell
+    orders at n = 4 nTotal <= 40: levels equal to the oracle's FP64 loop
+    cell by cell, E/Var within the mixed bar 1e-6."""
     b = native.Basis(2, [1, 2, 3], 5)
     m = meshes.naca_ogrid(64, 16)
     u0 = workloads.project_ic(workloads.euler2d_freestream_ic(), m.center, b)
@@ -123,7 +126,7 @@ def test_euler2d_levels_mixed_precision(oracle):
     for j in range(m.n_cells):
         lam0[j, 0] = oracle.DS(c, u0[j, 0])
     print("case: Euler2D NACA levels, mixed precision", flush=True)
-    keep = {}
+    keep = {"kernel": native.KERNEL_BLOCK}
     nt, nto, err = _run_levels_both(oracle, native.EULER2D, b, m, u0, lam0,
                                     dict(kind=native.EULER2D_PROBLEM, flux=native.FLUX_LAX_FRIEDRICHS, cfl=0.4),
                                     dict(epsilon=1e-6, precision=native.MIXED), 20, 2e-3, 4e-4, True, keep)

@@ -2,7 +2,7 @@
 each:
 
     python -m uqcreator_amd.configs c3 [--steps N]        2D Euler NACA 512x128, FP64 and mixed
-    python -m uqcreator_amd.configs c3-adaptive           the same with refinement levels 1/2/3
+    python -m uqcreator_amd.configs c3-adaptive           the same with refinement levels 1/2/3 (FP64, mixed)
     python -m uqcreator_amd.configs c4 [--steps N]        per-cell orders 1..6: the closure from
                                                           lambda_0, then N steps of the loop
     python -m uqcreator_amd.configs c5 [--steps N]        shallow-water dam break 1024^2
@@ -57,7 +57,9 @@ def c3(steps):
 
 
 def c3_adaptive():
-    _run_input("euler2d_naca_adaptive.toml", [], "c3_adaptive_2000")
+    for prec in ("fp64", "mixed"):
+        _run_input("euler2d_naca_adaptive.toml", [('precision = "fp64"', 'precision = "%s"' % prec)],
+                   "c3_adaptive_%s_2000" % prec)
 
 
 def _kernel_times(cl, s, steps):

@@ -116,8 +116,7 @@ int factor_mode(const uqk_ctx* ctx, bool per_cell_orders) {
         return uqk::newton_wave_reuse_fits(ctx->kind, ctx->M, ctx->Q, (int)ctx->max_lds) ? 1 : 0;
     if (ctx->kernel == UQK_KERNEL_WAVE) return 0;
     const bool block_ok = uqk::newton_block_fits(ctx->S, ctx->M, ctx->Q, (int)ctx->max_lds);
-    if (block_ok && (ctx->kernel == UQK_KERNEL_BLOCK ||
-                     (ctx->kernel == UQK_KERNEL_AUTO && (ctx->S * ctx->M >= 64 || per_cell_orders))))
+    if (block_ok && (ctx->kernel == UQK_KERNEL_BLOCK || (ctx->kernel == UQK_KERNEL_AUTO && ctx->S * ctx->M >= 64)))
         return 2;
     return 0;
 }
@@ -330,11 +329,7 @@ int launch_closure(uqk_ctx* ctx, int64_t n, const int32_t* nt, const double* u, 
         a.dt_cell = nullptr;
     }
     const bool block_ok = uqk::newton_block_fits(ctx->S, ctx->M, ctx->Q, (int)ctx->max_lds);
-    // (per-cell orders in mixed precision take the workgroup kernel at any n:
-    // it has their FP32 MFMA Hessian and kept factors, the generic one not)
-    const bool orders_mixed = nt && ctx->precision == UQK_MIXED;
-    if (ctx->kernel == UQK_KERNEL_BLOCK ||
-        (ctx->kernel == UQK_KERNEL_AUTO && (ctx->S * ctx->M >= 64 || orders_mixed) && block_ok)) {
+    if (ctx->kernel == UQK_KERNEL_BLOCK || (ctx->kernel == UQK_KERNEL_AUTO && ctx->S * ctx->M >= 64 && block_ok)) {
         if (!block_ok) return fail(ctx, -3, "closure too large for the workgroup kernel");
         if (a.pinv_mode == 1 && ctx->factor_mode != 2) a.pinv_mode = 0;  // (the slots hold inverses)
         const bool fuse = uq_out && u_out && ctx->c.max_iter != 1;
